@@ -1,0 +1,121 @@
+/*
+ * CPU ORACLE — test infrastructure only.
+ *
+ * A plain-C, FP64 restatement of the reference's geometric-multigrid solve
+ * (eljens/multigrid-poisson, include/ headers) used as the checker for the HIP
+ * library.  Only tests/, __graft_entry__.smoke() and bench.py's CPU-baseline
+ * leg may load it; the product never links, calls or falls back to it.
+ *
+ * Two modes, selected per solver:
+ *   OG_MODE_REFERENCE  — the reference's behaviour, quirks included, with the
+ *                        reference's exact arithmetic sequence (bit-identical
+ *                        to the reference built without FP contraction);
+ *   OG_MODE_CONSISTENT — the six requested faces honoured, homogeneous
+ *                        coarse-grid Neumann data, (i+j+k) colouring,
+ *                        cancellation-aware stencils with explicit fma(), and a
+ *                        ||r||/||f|| <= tol stop.
+ *
+ * How each function is pinned (details in tests/test_oracle.py):
+ *   - reference mode: against golden vectors recorded from the reference
+ *     itself (tests/golden/), bit for bit;
+ *   - consistent mode: closed forms (exactness of the 7-point stencil on
+ *     cubics, Jacobi eigen-damping, ghost-node Neumann identity), a dense
+ *     direct solve of the discrete system on tiny grids, and second-order
+ *     convergence against the analytic solution.
+ *
+ * Layout (reference include/array.h:84-94, 130-141): halo-padded C order,
+ * x slowest, z contiguous; idx(i,j,k) = (i+hW)*s0 + (j+hS)*s1 + (k+hB).
+ * Face and halo arrays are indexed in Location_t order (include/boundary.h:12):
+ * NORTH, SOUTH, EAST, WEST, TOP, BOTTOM.
+ */
+#ifndef MG_ORACLE_H
+#define MG_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { OG_N = 0, OG_S = 1, OG_E = 2, OG_W = 3, OG_T = 4, OG_B = 5 };
+enum { OG_DIRICHLET = 0, OG_NEUMANN = 1 };
+enum { OG_MODE_REFERENCE = 0, OG_MODE_CONSISTENT = 1 };
+enum { OG_JACOBI = 0, OG_GAUSS_SEIDEL = 1 };
+enum { OG_INJECTION = 0, OG_FULL_WEIGHTING = 1 };
+enum { OG_RELAXATION = 0, OG_VCYCLE = 1, OG_FCYCLE = 2 };
+
+typedef struct og_layout {
+    int64_t n[3];     /* points per axis, halos excluded */
+    int64_t halo[6];  /* Location_t order */
+    int64_t s[3];     /* element strides, s[2] == 1 */
+    int64_t size;     /* elements incl. halos */
+} og_layout;
+
+typedef struct og_params {
+    int64_t dims[3];
+    int64_t levels;
+    double h;           /* finest spacing (isotropic, src/parser.cpp:48) */
+    double origin[3];
+    int32_t bc[6];      /* requested BC per face, Location_t order */
+    int32_t mode;       /* OG_MODE_* */
+    int32_t relaxation; /* OG_JACOBI / OG_GAUSS_SEIDEL */
+    int32_t restriction;/* OG_INJECTION / OG_FULL_WEIGHTING */
+    int32_t problem;    /* analytic problem id for init(); <0: init_zero() */
+    int32_t threads;    /* OpenMP threads, 0 = runtime default */
+} og_params;
+
+/* ---- layout helpers ---- */
+void og_layout_make(og_layout* L, int64_t nx, int64_t ny, int64_t nz, const int64_t halo[6]);
+
+/* ---- analytic problems (same definitions the drivers use) ---- */
+double og_ufun(int problem, double x, double y, double z);
+double og_ffun(int problem, double x, double y, double z);
+double og_dudx(int problem, double x, double y, double z);
+double og_dudy(int problem, double x, double y, double z);
+double og_dudz(int problem, double x, double y, double z);
+
+/* ---- single operators on caller buffers (layouts given) ---- */
+/* box = {xmin,xmax,ymin,ymax,zmin,zmax} */
+void og_residual(int mode, const og_layout* L, const double* u, const double* f, double* r,
+                 const int64_t box[6], double h);
+void og_gs_half(int mode, const og_layout* L, double* u, const double* f, const int64_t box[6],
+                double omega, double h, int colour);
+void og_jacobi(int mode, const og_layout* L, double* u, const double* v, const double* f,
+               const int64_t box[6], double omega, double h);
+void og_restrict_inject(const og_layout* Li, const double* in, const og_layout* Lo, double* out);
+void og_restrict_fw(const og_layout* Li, const double* in, const og_layout* Lo, double* out);
+void og_prolong_trilinear(const og_layout* Li, const double* in, const og_layout* Lo, double* out);
+void og_bc_dirichlet(const og_layout* L, double* u, int loc, const double* g);
+void og_bc_neumann(const og_layout* L, double* u, int loc, const double* g, double h);
+/* coarse g = R(g_fine) - one-sided derivative of u_fine / h_fine (reference mode) */
+void og_bc_restrict_neumann(int restriction, const og_layout* Lf, const double* u_fine, int loc,
+                            const double* g_fine, double* g_coarse, double h_fine);
+double og_norm_inf(const og_layout* L, const double* a);
+
+/* ---- whole solver (one device, PoissonSolver<double,R,Trilinear,S>) ---- */
+typedef struct og_solver og_solver;
+og_solver* og_create(const og_params* p);
+void og_destroy(og_solver* s);
+void og_to_device(og_solver* s);          /* write_bc_to(u) (+uprev) on level 0..L-1 */
+/* one cycle (reference Vcycle/Fcycle from level 0) */
+void og_cycle(og_solver* s, int cycle, int64_t nsmooth, double omega);
+/* reference solve loop; relres_out receives the relative residual after each
+ * executed cycle (at most maxiter values); returns solve_iterations(). */
+int64_t og_solve(og_solver* s, int cycle, int64_t nsmooth, double omega, int64_t maxiter,
+                 int64_t miniter, double tol, double* relres_out, double* final_relres);
+/* one level's Relaxation::relax (boundary refresh + sweeps) and residual() */
+void og_relax_level(og_solver* s, int level, double omega, int64_t nsmooth);
+void og_residual_level(og_solver* s, int level);
+/* residual of level 0 (with boundary refresh) and its inf-norm / ||f||inf */
+double og_relres(og_solver* s);
+/* access: which = 'u','f','r','p' (uprev), face arrays 'g'+loc via og_face */
+double* og_array(og_solver* s, int level, int which);
+const og_layout* og_array_layout(og_solver* s, int level);
+double* og_face(og_solver* s, int level, int loc);
+const og_layout* og_face_layout(og_solver* s, int level, int loc);
+int32_t og_face_type(og_solver* s, int level, int loc);
+double og_level_h(og_solver* s, int level);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
