@@ -1,0 +1,246 @@
+/*
+ * mgp — C ABI of the MI355X (gfx950) geometric-multigrid library.
+ *
+ * Every entry point takes plain pointers and sizes, returns 0 on success and
+ * otherwise a hipError_t value or one of the MGP_ERR_* codes below, throws
+ * nothing, and keeps no pointer after it returns.  Work is enqueued on the
+ * `stream` argument (a hipStream_t; NULL = the library's default stream) and
+ * is asynchronous unless stated.
+ *
+ * Layout of every 3-D array (the reference's Array<T>, include/array.h:84-141
+ * of eljens/multigrid-poisson): halo-padded, C order, x slowest and z
+ * contiguous; element (i,j,k) of the shape sits at
+ *     (i + halo[WEST])*stride[0] + (j + halo[SOUTH])*stride[1] + (k + halo[BOTTOM])
+ * with i in [0,shape[0]) etc.  Face arrays of boundary data are dense arrays
+ * without halos of shape (nx,ny,1) for TOP/BOTTOM, (1,ny,nz) for EAST/WEST and
+ * (nx,1,nz) for NORTH/SOUTH (include/domain.h:125-190).
+ *
+ * Arithmetic modes:
+ *   MGP_MODE_REFERENCE   the reference's expression order and types, red/black
+ *                        colour = parity of the halo-inclusive linear index;
+ *   MGP_MODE_CONSISTENT  cancellation-aware stencils (sums of u_nb - u_c,
+ *                        explicit fma), colour = parity of i+j+k.
+ * Both are exact functions of their inputs: no atomics in the arithmetic, so
+ * results are reproducible bit for bit.
+ */
+#ifndef MGP_H
+#define MGP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ enums */
+#define MGP_OK 0
+#define MGP_ERR_ARG (-1)    /* invalid argument: shape mismatch, bad enum, ... */
+#define MGP_ERR_NODEV (-2)  /* no GPU visible */
+#define MGP_ERR_STATE (-3)  /* call out of order (e.g. graph end without begin) */
+
+enum { MGP_NORTH = 0, MGP_SOUTH = 1, MGP_EAST = 2, MGP_WEST = 3, MGP_TOP = 4, MGP_BOTTOM = 5 };
+enum { MGP_DIRICHLET = 0, MGP_NEUMANN = 1 };
+enum { MGP_MODE_REFERENCE = 0, MGP_MODE_CONSISTENT = 1 };
+enum { MGP_INJECTION = 0, MGP_FULL_WEIGHTING = 1 };
+enum { MGP_F64 = 0, MGP_F32 = 1 };
+
+typedef struct mgp_array {
+    void* data;        /* device pointer to the first element of the padded buffer */
+    int64_t shape[3];  /* points per axis (x, y, z), halos excluded */
+    int64_t halo[6];   /* halo width per face, order NORTH,SOUTH,EAST,WEST,TOP,BOTTOM */
+    int64_t stride[3]; /* element strides of x, y, z; stride[2] == 1 */
+} mgp_array;
+
+/* Boundary faces of one level, for kernels that form Neumann ghost values
+ * themselves (ghost = sign*2h*g + u(mirror), include/neumann.h:112-130). */
+typedef struct mgp_faces {
+    int32_t type[6];  /* effective type per face: MGP_DIRICHLET (boundary nodes fixed,
+                         outside the relaxation box) or MGP_NEUMANN */
+    const void* g[6]; /* device face array per Neumann face, NULL = homogeneous (g = 0) */
+    double h;         /* grid spacing of the level */
+} mgp_faces;
+
+/* box = {xmin, xmax, ymin, ymax, zmin, zmax}: half-open index ranges of the
+ * points a smoother or residual updates (include/relaxation.h:78-83). */
+
+/* ----------------------------------------------------- device and memory */
+int mgp_device_count(int* count);
+int mgp_set_device(int device);
+int mgp_malloc(void** ptr, size_t bytes);
+int mgp_free(void* ptr);
+int mgp_host_alloc(void** ptr, size_t bytes); /* pinned host memory */
+int mgp_host_free(void* ptr);
+int mgp_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int mgp_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int mgp_memcpy_d2d(void* dst, const void* src, size_t bytes, void* stream);
+int mgp_stream_create(void** stream);
+int mgp_stream_destroy(void* stream);
+int mgp_stream_sync(void* stream);
+int mgp_mem_info(size_t* free_bytes, size_t* total_bytes);
+
+/* ------------------------------------------- elementwise (replaces the
+ * reference's DeviceArray::init_zero/add/infinity_norm, devicearray.h:132-240) */
+/* every element of the padded buffer (halos included) = value; a kernel, not a
+ * memset, so it can be captured in a graph */
+int mgp_fill_f64(const mgp_array* a, double value, void* stream);
+int mgp_fill_f32(const mgp_array* a, double value, void* stream);
+/* a += b over the shape (devicearray.h:161-196) */
+int mgp_add_f64(const mgp_array* a, const mgp_array* b, void* stream);
+int mgp_add_f32(const mgp_array* a, const mgp_array* b, void* stream);
+/* *norm_dev = max(*norm_dev, max |a| over the shape) (devicearray.h:198-240);
+ * norm_dev is a device double the caller zeroes with mgp_zero_scalars */
+int mgp_norm_inf_f64(const mgp_array* a, double* norm_dev, void* stream);
+int mgp_norm_inf_f32(const mgp_array* a, double* norm_dev, void* stream);
+int mgp_zero_scalars(double* dev, int64_t n, void* stream);
+
+/* ------------------------------------------------------------ smoothers */
+/* One red (colour 0) or black (colour 1) half-sweep in place over the box:
+ * u = (1-w)u + w/6 (sum_nb u - h^2 f) (include/gaussseidel.h:28-94).  Halos
+ * must hold current ghost values (see mgp_bc_neumann_*). */
+int mgp_rbgs_half_f64(int mode, const mgp_array* u, const mgp_array* f, const int64_t box[6], double omega,
+                      double h, int colour, void* stream);
+int mgp_rbgs_half_f32(int mode, const mgp_array* u, const mgp_array* f, const int64_t box[6], double omega,
+                      double h, int colour, void* stream);
+/* One full red-then-black sweep in one pass, out of place: u_out = GS(u_in).
+ * Neumann ghosts are formed on the fly from `faces`, as if refreshed before
+ * each half-sweep (include/relaxation.h:35-52).  Points outside the box are not
+ * written (u_out must already hold the boundary values).  Requires a
+ * checkerboard colouring: always in consistent mode; in reference mode only
+ * when stride[0] and stride[1] are odd (MGP_ERR_ARG otherwise). */
+int mgp_rbgs_sweep_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                       const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+int mgp_rbgs_sweep_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                       const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+/* Weighted Jacobi over the box: u = (1-w)v + w/6 (sum_nb v - h^2 f)
+ * (include/jacobi.h:39-82).  v's halos must hold current ghost values. */
+int mgp_jacobi_f64(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t box[6],
+                   double omega, double h, void* stream);
+int mgp_jacobi_f32(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t box[6],
+                   double omega, double h, void* stream);
+
+/* ------------------------------------------------------------- residual */
+/* r = f - L_h u over the box (include/residual.h:58-79); u's halos must hold
+ * current ghost values.  If norm_dev != NULL, also *norm_dev = max(*norm_dev,
+ * max |r| over r's whole shape) in the same pass. */
+int mgp_residual_f64(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
+                     double h, double* norm_dev, void* stream);
+int mgp_residual_f32(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
+                     double h, double* norm_dev, void* stream);
+
+/* ----------------------------------------------------- grid transfers */
+/* out(i,j,k) = in(2i,2j,2k) over out's shape (include/injection.h:29-67) */
+int mgp_restrict_inject_f64(const mgp_array* in, const mgp_array* out, void* stream);
+int mgp_restrict_inject_f32(const mgp_array* in, const mgp_array* out, void* stream);
+/* 27-point (1,2,1)^3/64 full weighting with neighbour indices clamped to the
+ * shape (include/fullweighting.h:37-122); also valid on face arrays */
+int mgp_restrict_fw_f64(const mgp_array* in, const mgp_array* out, void* stream);
+int mgp_restrict_fw_f32(const mgp_array* in, const mgp_array* out, void* stream);
+/* fine = trilinear interpolation of coarse over fine's whole shape
+ * (include/trilinearinterpolation.h:29-134), bit-identical to its four passes */
+int mgp_prolong_trilinear_f64(const mgp_array* coarse, const mgp_array* fine, void* stream);
+int mgp_prolong_trilinear_f32(const mgp_array* coarse, const mgp_array* fine, void* stream);
+/* u += trilinear interpolation of coarse, over u's whole shape, in one pass:
+ * the reference's prolong-into-scratch followed by DeviceArray::add
+ * (include/vcycle.h:81-94) without the scratch round trip */
+int mgp_prolong_add_f64(const mgp_array* coarse, const mgp_array* u, void* stream);
+int mgp_prolong_add_f32(const mgp_array* coarse, const mgp_array* u, void* stream);
+
+/* ------------------------------------------------- boundary conditions */
+/* Dirichlet: boundary nodes of face `loc` = g (include/dirichlet.h:50-93) */
+int mgp_bc_dirichlet_f64(const mgp_array* u, int loc, const void* g, void* stream);
+int mgp_bc_dirichlet_f32(const mgp_array* u, int loc, const void* g, void* stream);
+/* Neumann: ghost layer of face `loc` = sign*2h*g + u(mirror), sign -1 on
+ * WEST/SOUTH/BOTTOM (include/neumann.h:63-131); g NULL = homogeneous */
+int mgp_bc_neumann_f64(const mgp_array* u, int loc, const void* g, double h, void* stream);
+int mgp_bc_neumann_f32(const mgp_array* u, int loc, const void* g, double h, void* stream);
+/* Coarse Neumann data as the reference forms it (include/neumann.h:146-252):
+ * g_coarse = R(g_fine) - (c1 u(b) + c2 u(b+-1) + c3 u(b+-2))/h_fine with
+ * c = (-3/2, 2, -1/2) on WEST/SOUTH/BOTTOM and (3/2, -2, 1/2) elsewhere;
+ * restriction = MGP_INJECTION or MGP_FULL_WEIGHTING */
+int mgp_bc_restrict_neumann_f64(int restriction, const mgp_array* u_fine, int loc, const void* g_fine,
+                                void* g_coarse, double h_fine, void* stream);
+int mgp_bc_restrict_neumann_f32(int restriction, const mgp_array* u_fine, int loc, const void* g_fine,
+                                void* g_coarse, double h_fine, void* stream);
+
+/* -------------------------------------------------------- graphs, timing */
+/* Record the work enqueued on `stream` between begin and end into a graph
+ * (hipStreamBeginCapture, relaxed mode); *exec receives an instantiated graph */
+int mgp_graph_begin(void* stream);
+int mgp_graph_end(void* stream, void** exec);
+int mgp_graph_launch(void* exec, void* stream);
+int mgp_graph_destroy(void* exec);
+int mgp_event_create(void** event);
+int mgp_event_destroy(void* event);
+int mgp_event_record(void* event, void* stream);
+int mgp_event_sync(void* event);
+int mgp_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* --------------------------------------------------------- whole solver
+ * A PoissonSolver<T, R, TrilinearInterpolation, S> (include/libpoisson.h:28-67
+ * of the reference) behind an opaque handle: the calls an FFI binding of the
+ * reference's solver façade makes.  Host arrays returned by
+ * mgp_solver_host_array are the solver's own mirrors (Array<T>::at): write
+ * them before mgp_solver_to_device, read them after mgp_solver_to_host. */
+enum { MGP_JACOBI = 0, MGP_GAUSS_SEIDEL = 1 };
+#define MGP_SOLVER_NO_GRAPHS 1 /* run cycles launch by launch (Settings::use_graphs = false) */
+#define MGP_SOLVER_NO_FUSED 2  /* red-black half-sweeps only (Settings::fused_sweeps = false) */
+#define MGP_SOLVER_QUIET 4     /* no per-cycle timing line (Settings::print_cycle_time = false) */
+
+typedef struct mgp_solver_config {
+    int64_t dims[3];     /* Settings::dims (include/settings.h:17) */
+    int64_t levels;      /* Settings::levels */
+    double h;            /* Settings::h */
+    double origin[3];    /* Settings::origin */
+    int32_t bc[6];       /* BoundaryCondition, Location_t order N,S,E,W,T,B: MGP_DIRICHLET / MGP_NEUMANN */
+    int32_t dtype;       /* T: MGP_F64 / MGP_F32 */
+    int32_t restriction; /* R: MGP_INJECTION / MGP_FULL_WEIGHTING */
+    int32_t relaxation;  /* S: MGP_JACOBI / MGP_GAUSS_SEIDEL */
+    int32_t mode;        /* MGP_MODE_REFERENCE / MGP_MODE_CONSISTENT */
+    int32_t flags;       /* MGP_SOLVER_* */
+    int64_t maxiter;     /* Settings::maxiter */
+    int64_t miniter;     /* Settings::miniter */
+    double tolerance;    /* Settings::tolerance */
+} mgp_solver_config;
+
+/* PoissonSolver(settings, BC) (libpoisson.h:79-87) */
+int mgp_solver_create(const mgp_solver_config* cfg, void** solver);
+int mgp_solver_destroy(void* solver);
+/* init() with a built-in analytic problem (0: sin(2 pi x)sin(2 pi y)sin(2 pi z);
+ * 1-3: the reference's PROBLEM1-3, src/problem_definition.cpp) (libpoisson.h:159-171) */
+int mgp_solver_init_problem(void* solver, int problem);
+/* init() with caller functions u, f, du/dx, du/dy, du/dz */
+int mgp_solver_init_fn(void* solver, double (*u)(double, double, double), double (*f)(double, double, double),
+                       double (*dudx)(double, double, double), double (*dudy)(double, double, double),
+                       double (*dudz)(double, double, double));
+int mgp_solver_init_zero(void* solver);  /* libpoisson.h:174-190 */
+int mgp_solver_to_device(void* solver);  /* libpoisson.h:193-209 */
+int mgp_solver_to_host(void* solver);    /* libpoisson.h:212-228 */
+/* solve(solver, nsmooth, omega, "", maxtime) (libpoisson.h:237-357);
+ * solver = "Vcycle"/"vcycle"/"Fcycle"/"fcycle"/"Relaxation"/"relaxation" */
+int mgp_solver_solve(void* solver, const char* cycle, int64_t nsmooth, double omega, double maxtime);
+/* one cycle without norms, enqueued; then mgp_solver_relres for ||r||/||f|| */
+int mgp_solver_cycle(void* solver, const char* cycle, int64_t nsmooth, double omega);
+int mgp_solver_relres(void* solver, double* relres);
+int mgp_solver_sync(void* solver);
+/* relative_residual(), solve_time(), solve_iterations() (libpoisson.h:378-393) */
+int mgp_solver_stats(void* solver, double* rel_res, double* seconds, int64_t* iterations);
+/* host mirror and device descriptor of level `level`'s u ('u'), uprev ('p'),
+ * f ('f') or r ('r') */
+int mgp_solver_array(void* solver, int level, int which, void** host, mgp_array* device);
+/* host mirror and device descriptor of a face array; *type receives the
+ * effective boundary type (MGP_DIRICHLET / MGP_NEUMANN) */
+int mgp_solver_face(void* solver, int level, int loc, void** host, mgp_array* device, int* type);
+/* VTK of level 0's u (save, libpoisson.h:359-363) */
+int mgp_solver_save(void* solver, const char* path);
+/* message of the last failing mgp_solver_* call on this thread */
+const char* mgp_last_error(void);
+
+/* ---------------------------------------------------------------- info */
+const char* mgp_error_string(int code);
+const char* mgp_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MGP_H */

@@ -1,0 +1,785 @@
+// gfx950 kernels of the mgp library and their C entry points.
+//
+// Each kernel restates one OpenMP target region of the reference
+// (eljens/multigrid-poisson, include/*.h) as a HIP kernel; the cited
+// file:line is the region it replaces.  Reference-mode expressions keep the
+// reference's operand order and C++ types term by term (this file is built
+// with -ffp-contract=off), so results are bit-identical to the reference's
+// host build.  Consistent-mode expressions are this library's own
+// cancellation-aware forms with explicit fma.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "mgp.h"
+#include "mgp_device.h"
+
+namespace mgp {
+
+struct Box {
+    int64_t x0, x1, y0, y1, z0, z1;
+};
+
+static inline Box box_of(const int64_t b[6]) { return Box{b[0], b[1], b[2], b[3], b[4], b[5]}; }
+static inline bool box_empty(const Box& b) { return b.x1 <= b.x0 || b.y1 <= b.y0 || b.z1 <= b.z0; }
+
+// Launch geometry over a box: 64 threads along z (contiguous), 4 rows of y;
+// grid.x enumerates (x, z-chunk) pairs, grid.y the y-row groups.
+constexpr int BZ = 64;
+constexpr int BY = 4;
+
+struct BoxGrid {
+    dim3 grid, block;
+    int64_t zchunks;
+};
+
+static inline BoxGrid box_grid(const Box& b) {
+    BoxGrid g;
+    g.zchunks = (b.z1 - b.z0 + BZ - 1) / BZ;
+    g.grid = dim3((unsigned)((b.x1 - b.x0) * g.zchunks), (unsigned)((b.y1 - b.y0 + BY - 1) / BY), 1);
+    g.block = dim3(BZ, BY, 1);
+    return g;
+}
+
+// (i, j, k) of this thread for a BoxGrid launch; false when outside the box.
+__device__ __forceinline__ bool box_point(const Box& b, int64_t zchunks, int64_t& i, int64_t& j, int64_t& k) {
+    const int64_t bx = blockIdx.x;
+    i = b.x0 + bx / zchunks;
+    k = b.z0 + (bx % zchunks) * BZ + threadIdx.x;
+    j = b.y0 + (int64_t)blockIdx.y * BY + threadIdx.y;
+    return k < b.z1 && j < b.y1;
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Max over the block of non-negative v, folded into *out with a 64-bit atomic
+// max on the IEEE bit pattern (order-independent, so the result is exact).
+__device__ __forceinline__ void block_max_into(double v, unsigned long long* out) {
+    __shared__ double part[16];
+    const int lane = (threadIdx.y * blockDim.x + threadIdx.x) & 63;
+    const int wave = (threadIdx.y * blockDim.x + threadIdx.x) >> 6;
+    const int nwaves = (blockDim.x * blockDim.y + 63) >> 6;
+    v = wave_max(v);
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    if (wave == 0) {
+        double w = lane < nwaves ? part[lane] : 0.0;
+        w = wave_max(w);
+        if (lane == 0 && w > 0.0) atomicMax(out, (unsigned long long)__double_as_longlong(w));
+    }
+}
+
+// ------------------------------------------------------------ elementwise
+
+template <class T>
+__global__ void k_fill(T* p, int64_t n, T v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) p[t] = v;
+}
+
+// include/devicearray.h:175-194: a(i,j,k) += b(i,j,k) over the shape
+template <class T>
+__global__ void k_add(View<T> a, View<T> b, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    a.at(i, j, k) += b.at(i, j, k);
+}
+
+// include/devicearray.h:211-237: max |a| over the shape
+template <class T>
+__global__ void k_norm(View<T> a, Box bx, int64_t zc, unsigned long long* out) {
+    int64_t i, j, k;
+    double v = 0.0;
+    if (box_point(bx, zc, i, j, k)) v = fabs((double)a.at(i, j, k));
+    block_max_into(v, out);
+}
+
+__global__ void k_zero_scalars(double* p, int64_t n) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) p[t] = 0.0;
+}
+
+// ----------------------------------------------------------- smoothers
+
+// Colour of (i,j,k): reference = parity of the halo-inclusive linear index
+// (include/gaussseidel.h:56,81; `lin0` is the linear index of (0,0,0));
+// consistent = parity of i+j+k.
+template <int MODE>
+__device__ __forceinline__ int colour_of(int64_t lin0, int64_t s0, int64_t s1, int64_t i, int64_t j, int64_t k) {
+    if (MODE == MGP_MODE_REFERENCE) return (int)((lin0 + i * s0 + j * s1 + k) & 1);
+    return (int)((i + j + k) & 1);
+}
+
+// One smoother update of point c from the current values around it.
+// Reference: include/gaussseidel.h:58-60 / jacobi.h:66-69
+//   (1-w) v + w/6 (vW + vE + vS + vN + vB + vT - h^2 f)
+// Consistent: v + w/6 * fma(-h^2, f, sum(v_nb - v))
+template <class T, int MODE>
+__device__ __forceinline__ T relax_point(T vc, T vW, T vE, T vS, T vN, T vB, T vT, T fc, T one_omega, T omega_sixth,
+                                         T hsq) {
+    if (MODE == MGP_MODE_REFERENCE) {
+        return one_omega * vc + omega_sixth * (vW + vE + vS + vN + vB + vT - hsq * fc);
+    } else {
+        const T s = ((vW - vc) + (vE - vc)) + ((vS - vc) + (vN - vc)) + ((vB - vc) + (vT - vc));
+        return fma(omega_sixth, fma(-hsq, fc, s), vc);
+    }
+}
+
+// include/gaussseidel.h:28-94: one half-sweep in place, neighbours read from
+// memory (the halo holds the ghosts written just before)
+template <class T, int MODE>
+__global__ void k_gs_half(View<T> u, View<T> f, Box bx, int64_t zc, T one_omega, T omega_sixth, T hsq, int colour,
+                          int64_t lin0) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    if (colour_of<MODE>(lin0, u.s0, u.s1, i, j, k) != colour) return;
+    const int64_t c = u.ix(i, j, k);
+    T* p = u.p;
+    const T fc = f.at(i, j, k);
+    p[c] = relax_point<T, MODE>(p[c], p[c - u.s0], p[c + u.s0], p[c - u.s1], p[c + u.s1], p[c - 1], p[c + 1], fc,
+                                one_omega, omega_sixth, hsq);
+}
+
+// include/jacobi.h:39-82: u = J(v), out of place; the reference writes
+// u = (1-w)v and then adds, which is the same two roundings
+template <class T, int MODE>
+__global__ void k_jacobi(View<T> u, View<T> v, View<T> f, Box bx, int64_t zc, T one_omega, T omega_sixth, T hsq) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const int64_t c = v.ix(i, j, k);
+    const T* p = v.p;
+    u.at(i, j, k) = relax_point<T, MODE>(p[c], p[c - v.s0], p[c + v.s0], p[c - v.s1], p[c + v.s1], p[c - 1], p[c + 1],
+                                         f.at(i, j, k), one_omega, omega_sixth, hsq);
+}
+
+// Red pass of an out-of-place sweep: red points of u_out = GS update from
+// u_in, black points copied; ghosts formed on the fly from u_in.
+template <class T, int MODE>
+__global__ void k_sweep_red(View<T> ui, View<T> uo, View<T> f, Faces<T> F, Box bx, int64_t zc, T one_omega,
+                            T omega_sixth, T hsq, int64_t lin0) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    if (colour_of<MODE>(lin0, ui.s0, ui.s1, i, j, k) != 0) {
+        uo.at(i, j, k) = ui.at(i, j, k);
+        return;
+    }
+    const T vW = neighbour(ui, F, i, j, k, 0, -1), vE = neighbour(ui, F, i, j, k, 0, 1);
+    const T vS = neighbour(ui, F, i, j, k, 1, -1), vN = neighbour(ui, F, i, j, k, 1, 1);
+    const T vB = neighbour(ui, F, i, j, k, 2, -1), vT = neighbour(ui, F, i, j, k, 2, 1);
+    uo.at(i, j, k) = relax_point<T, MODE>(ui.at(i, j, k), vW, vE, vS, vN, vB, vT, f.at(i, j, k), one_omega,
+                                          omega_sixth, hsq);
+}
+
+// Black pass in place on u (its red neighbours are not written here).
+template <class T, int MODE>
+__global__ void k_sweep_black(View<T> u, View<T> f, Faces<T> F, Box bx, int64_t zc, T one_omega, T omega_sixth, T hsq,
+                              int64_t lin0) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    if (colour_of<MODE>(lin0, u.s0, u.s1, i, j, k) != 1) return;
+    const T vW = neighbour(u, F, i, j, k, 0, -1), vE = neighbour(u, F, i, j, k, 0, 1);
+    const T vS = neighbour(u, F, i, j, k, 1, -1), vN = neighbour(u, F, i, j, k, 1, 1);
+    const T vB = neighbour(u, F, i, j, k, 2, -1), vT = neighbour(u, F, i, j, k, 2, 1);
+    u.at(i, j, k) = relax_point<T, MODE>(u.at(i, j, k), vW, vE, vS, vN, vB, vT, f.at(i, j, k), one_omega, omega_sixth,
+                                         hsq);
+}
+
+// ------------------------------------------------------------- residual
+
+// include/residual.h:58-79.  Reference: r = f - (sum_nb - 6.0*u)/hsq with the
+// literal 6.0 a double, as in the reference.  Consistent:
+// r = fma(-1/h^2, sum(u_nb - u), f).
+template <class T, int MODE>
+__device__ __forceinline__ T residual_point(const View<T>& u, int64_t c, T fc, T hsq, T inv_h2) {
+    const T* p = u.p;
+    if (MODE == MGP_MODE_REFERENCE) {
+        return fc - (p[c - u.s0] + p[c + u.s0] + p[c - u.s1] + p[c + u.s1] + p[c - 1] + p[c + 1] - 6.0 * p[c]) / hsq;
+    } else {
+        const T uc = p[c];
+        const T s = ((p[c - u.s0] - uc) + (p[c + u.s0] - uc)) + ((p[c - u.s1] - uc) + (p[c + u.s1] - uc)) +
+                    ((p[c - 1] - uc) + (p[c + 1] - uc));
+        return fma(-inv_h2, s, fc);
+    }
+}
+
+template <class T, int MODE>
+__global__ void k_residual(View<T> u, View<T> f, View<T> r, Box bx, int64_t zc, T hsq, T inv_h2) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    r.at(i, j, k) = residual_point<T, MODE>(u, u.ix(i, j, k), f.at(i, j, k), hsq, inv_h2);
+}
+
+// Residual on the box plus max |r| over r's whole shape in the same pass
+// (the grid covers the shape; points outside the box only contribute).
+template <class T, int MODE>
+__global__ void k_residual_norm(View<T> u, View<T> f, View<T> r, Box shape, int64_t zc, Box bx, T hsq, T inv_h2,
+                                unsigned long long* out) {
+    int64_t i, j, k;
+    double v = 0.0;
+    if (box_point(shape, zc, i, j, k)) {
+        T rv;
+        if (i >= bx.x0 && i < bx.x1 && j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1) {
+            rv = residual_point<T, MODE>(u, u.ix(i, j, k), f.at(i, j, k), hsq, inv_h2);
+            r.at(i, j, k) = rv;
+        } else {
+            rv = r.at(i, j, k);
+        }
+        v = fabs((double)rv);
+    }
+    block_max_into(v, out);
+}
+
+// ------------------------------------------------------- grid transfers
+
+// include/injection.h:47-66
+template <class T>
+__global__ void k_inject(View<T> in, View<T> out, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    out.at(i, j, k) = in.at(2 * i, 2 * j, 2 * k);
+}
+
+// include/fullweighting.h:59-121: neighbours clamped to [0, n-1]; the three
+// z-planes summed in the reference's term order (T tmp = plane; tmp += plane)
+template <class T>
+__global__ void k_fw(View<T> in, View<T> out, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const int64_t n0 = in.n0, n1 = in.n1, n2 = in.n2;
+    const int64_t ii = min(n0 - 1, 2 * i), jj = min(n1 - 1, 2 * j), kk = min(n2 - 1, 2 * k);
+    const int64_t im = max((int64_t)0, ii - 1), ip = min(n0 - 1, ii + 1);
+    const int64_t jm = max((int64_t)0, jj - 1), jp = min(n1 - 1, jj + 1);
+    const int64_t km = max((int64_t)0, kk - 1), kp = min(n2 - 1, kk + 1);
+#define A(a, b, c) in.at(a, b, c)
+    T tmp = 1.0 * A(im, jm, km) + 2.0 * A(im, jj, km) + 1.0 * A(im, jp, km) + 2.0 * A(ii, jm, km) +
+            4.0 * A(ii, jj, km) + 2.0 * A(ii, jp, km) + 1.0 * A(ip, jm, km) + 2.0 * A(ip, jj, km) +
+            1.0 * A(ip, jp, km);
+    tmp += 2.0 * A(im, jm, kk) + 4.0 * A(im, jj, kk) + 2.0 * A(im, jp, kk) + 4.0 * A(ii, jm, kk) +
+           8.0 * A(ii, jj, kk) + 4.0 * A(ii, jp, kk) + 2.0 * A(ip, jm, kk) + 4.0 * A(ip, jj, kk) +
+           2.0 * A(ip, jp, kk);
+    tmp += 1.0 * A(im, jm, kp) + 2.0 * A(im, jj, kp) + 1.0 * A(im, jp, kp) + 2.0 * A(ii, jm, kp) +
+           4.0 * A(ii, jj, kp) + 2.0 * A(ii, jp, kp) + 1.0 * A(ip, jm, kp) + 2.0 * A(ip, jj, kp) +
+           1.0 * A(ip, jp, kp);
+#undef A
+    out.at(i, j, k) = tmp / 64.0;
+}
+
+// Trilinear value at fine point (i,j,k), composed exactly as the reference's
+// four passes compose it (include/trilinearinterpolation.h:50-133): injection
+// at even points, then x-, y- and z-midpoints, each 0.5*(left + right) of the
+// previous passes' values.
+template <class T>
+__device__ __forceinline__ T half_sum(T a, T b) {
+    return 0.5 * (a + b);
+}
+
+template <class T>
+__device__ __forceinline__ T tri_xy(const View<T>& c, int64_t a, int64_t b, int64_t cc, int pi, int pj) {
+    if (!pi && !pj) return c.at(a, b, cc);
+    if (pi && !pj) return half_sum<T>(c.at(a, b, cc), c.at(a + 1, b, cc));
+    if (!pi && pj) return half_sum<T>(c.at(a, b, cc), c.at(a, b + 1, cc));
+    return half_sum<T>(half_sum<T>(c.at(a, b, cc), c.at(a + 1, b, cc)),
+                       half_sum<T>(c.at(a, b + 1, cc), c.at(a + 1, b + 1, cc)));
+}
+
+template <class T>
+__device__ __forceinline__ T trilinear(const View<T>& c, int64_t i, int64_t j, int64_t k) {
+    const int pi = (int)(i & 1), pj = (int)(j & 1), pk = (int)(k & 1);
+    const int64_t a = i >> 1, b = j >> 1, cc = k >> 1;
+    const T lo = tri_xy<T>(c, a, b, cc, pi, pj);
+    if (!pk) return lo;
+    return half_sum<T>(lo, tri_xy<T>(c, a, b, cc + 1, pi, pj));
+}
+
+template <class T, bool ADD>
+__global__ void k_prolong(View<T> coarse, View<T> fine, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const T v = trilinear<T>(coarse, i, j, k);
+    if (ADD)
+        fine.at(i, j, k) += v;  // include/devicearray.h:190
+    else
+        fine.at(i, j, k) = v;
+}
+
+// --------------------------------------------------- boundary conditions
+
+// Offsets of face `loc` in the domain: the face array's (i,j,k) maps to the
+// boundary node (i+o0, j+o1, k+o2) (include/dirichlet.h:53-68)
+__device__ __forceinline__ void face_geometry(int loc, int64_t n0, int64_t n1, int64_t n2, int64_t& f0, int64_t& f1,
+                                              int64_t& f2) {
+    f0 = (loc == MGP_EAST || loc == MGP_WEST) ? 1 : n0;
+    f1 = (loc == MGP_NORTH || loc == MGP_SOUTH) ? 1 : n1;
+    f2 = (loc == MGP_TOP || loc == MGP_BOTTOM) ? 1 : n2;
+}
+
+template <class T>
+__global__ void k_bc_dirichlet(View<T> u, int loc, const T* g) {
+    int64_t f0, f1, f2;
+    face_geometry(loc, u.n0, u.n1, u.n2, f0, f1, f2);
+    const int64_t n = f0 * f1 * f2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = t / (f1 * f2), j = (t / f2) % f1, k = t % f2;
+    const int64_t o0 = loc == MGP_EAST ? u.n0 - 1 : 0;
+    const int64_t o1 = loc == MGP_NORTH ? u.n1 - 1 : 0;
+    const int64_t o2 = loc == MGP_TOP ? u.n2 - 1 : 0;
+    u.at(i + o0, j + o1, k + o2) = g[t];
+}
+
+// include/neumann.h:63-131
+template <class T>
+__global__ void k_bc_neumann(View<T> u, int loc, const T* g, T two_h) {
+    int64_t f0, f1, f2;
+    face_geometry(loc, u.n0, u.n1, u.n2, f0, f1, f2);
+    const int64_t n = f0 * f1 * f2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = t / (f1 * f2), j = (t / f2) % f1, k = t % f2;
+    int64_t a = i, b = j, c = k, d0 = 0, d1 = 0, d2 = 0;
+    T sign = 1.0;
+    switch (loc) {
+    case MGP_EAST: a = u.n0; d0 = -2; break;
+    case MGP_WEST: a = -1; d0 = 2; sign = -1.0; break;
+    case MGP_NORTH: b = u.n1; d1 = -2; break;
+    case MGP_SOUTH: b = -1; d1 = 2; sign = -1.0; break;
+    case MGP_TOP: c = u.n2; d2 = -2; break;
+    default: c = -1; d2 = 2; sign = -1.0; break;
+    }
+    const T gv = g ? g[t] : (T)0.0;
+    u.at(a, b, c) = sign * two_h * gv + u.at(a + d0, b + d1, c + d2);
+}
+
+// include/neumann.h:146-252: coarse face data = R(fine face data), then minus
+// the one-sided difference of the fine solution at the face divided by h_fine.
+template <class T>
+__global__ void k_bc_restrict_neumann(int restriction, View<T> uf, int loc, View<T> gf, View<T> gc, T c1, T c2, T c3,
+                                      T hf) {
+    const int64_t n = gc.n0 * gc.n1 * gc.n2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t i = t / (gc.n1 * gc.n2), j = (t / gc.n2) % gc.n1, k = t % gc.n2;
+    T v;
+    if (restriction == MGP_FULL_WEIGHTING) {
+        const int64_t n0 = gf.n0, n1 = gf.n1, n2 = gf.n2;
+        const int64_t ii = min(n0 - 1, 2 * i), jj = min(n1 - 1, 2 * j), kk = min(n2 - 1, 2 * k);
+        const int64_t im = max((int64_t)0, ii - 1), ip = min(n0 - 1, ii + 1);
+        const int64_t jm = max((int64_t)0, jj - 1), jp = min(n1 - 1, jj + 1);
+        const int64_t km = max((int64_t)0, kk - 1), kp = min(n2 - 1, kk + 1);
+#define A(a, b, c) gf.at(a, b, c)
+        T tmp = 1.0 * A(im, jm, km) + 2.0 * A(im, jj, km) + 1.0 * A(im, jp, km) + 2.0 * A(ii, jm, km) +
+                4.0 * A(ii, jj, km) + 2.0 * A(ii, jp, km) + 1.0 * A(ip, jm, km) + 2.0 * A(ip, jj, km) +
+                1.0 * A(ip, jp, km);
+        tmp += 2.0 * A(im, jm, kk) + 4.0 * A(im, jj, kk) + 2.0 * A(im, jp, kk) + 4.0 * A(ii, jm, kk) +
+               8.0 * A(ii, jj, kk) + 4.0 * A(ii, jp, kk) + 2.0 * A(ip, jm, kk) + 4.0 * A(ip, jj, kk) +
+               2.0 * A(ip, jp, kk);
+        tmp += 1.0 * A(im, jm, kp) + 2.0 * A(im, jj, kp) + 1.0 * A(im, jp, kp) + 2.0 * A(ii, jm, kp) +
+               4.0 * A(ii, jj, kp) + 2.0 * A(ii, jp, kp) + 1.0 * A(ip, jm, kp) + 2.0 * A(ip, jj, kp) +
+               1.0 * A(ip, jp, kp);
+#undef A
+        v = tmp / 64.0;
+    } else {
+        v = gf.at(2 * i, 2 * j, 2 * k);
+    }
+    int64_t o0 = 0, o1 = 0, o2 = 0, d0 = 0, d1 = 0, d2 = 0;
+    switch (loc) {
+    case MGP_EAST: o0 = uf.n0 / 2; d0 = -1; break;
+    case MGP_WEST: d0 = 1; break;
+    case MGP_NORTH: o1 = uf.n1 / 2; d1 = -1; break;
+    case MGP_SOUTH: d1 = 1; break;
+    case MGP_TOP: o2 = uf.n2 / 2; d2 = -1; break;
+    default: d2 = 1; break;
+    }
+    const int64_t a = 2 * (i + o0), b = 2 * (j + o1), c = 2 * (k + o2);
+    v -= (c1 * uf.at(a, b, c) + c2 * uf.at(a + d0, b + d1, c + d2) + c3 * uf.at(a + 2 * d0, b + 2 * d1, c + 2 * d2)) /
+         hf;
+    gc.at(i, j, k) = v;
+}
+
+// ======================================================= host-side wrappers
+
+static inline int launch_status() {
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+
+static inline bool valid_array(const mgp_array* a) {
+    if (!a || !a->data) return false;
+    for (int d = 0; d < 3; d++)
+        if (a->shape[d] < 1) return false;
+    for (int q = 0; q < 6; q++)
+        if (a->halo[q] < 0) return false;
+    return a->stride[2] == 1 && a->stride[1] >= a->shape[2] + a->halo[MGP_TOP] + a->halo[MGP_BOTTOM] &&
+           a->stride[0] >= a->stride[1] * (a->shape[1] + a->halo[MGP_NORTH] + a->halo[MGP_SOUTH]);
+}
+
+static inline bool same_shape(const mgp_array* a, const mgp_array* b) {
+    return a->shape[0] == b->shape[0] && a->shape[1] == b->shape[1] && a->shape[2] == b->shape[2];
+}
+
+static inline bool valid_box(const mgp_array* a, const int64_t b[6]) {
+    // the stencil reads one point beyond the box on each side: inside the
+    // shape, or in a halo layer that exists
+    const int64_t lo[3] = {a->halo[MGP_WEST], a->halo[MGP_SOUTH], a->halo[MGP_BOTTOM]};
+    const int64_t hi[3] = {a->halo[MGP_EAST], a->halo[MGP_NORTH], a->halo[MGP_TOP]};
+    for (int d = 0; d < 3; d++) {
+        if (b[2 * d] >= b[2 * d + 1]) continue;
+        if (b[2 * d] - 1 < -lo[d] || b[2 * d + 1] > a->shape[d] + hi[d]) return false;
+    }
+    return true;
+}
+
+static inline int64_t lin0_of(const mgp_array* a) {
+    return a->halo[MGP_WEST] * a->stride[0] + a->halo[MGP_SOUTH] * a->stride[1] + a->halo[MGP_BOTTOM];
+}
+
+static inline Box shape_box(const mgp_array* a) { return Box{0, a->shape[0], 0, a->shape[1], 0, a->shape[2]}; }
+
+template <class T>
+static int fill(const mgp_array* a, double value, void* stream) {
+    if (!a || !a->data) return MGP_ERR_ARG;
+    const int64_t n = a->stride[0] * (a->shape[0] + a->halo[MGP_EAST] + a->halo[MGP_WEST]);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill<T>, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (T*)a->data, n, (T)value);
+    return launch_status();
+}
+
+template <class T>
+static int add(const mgp_array* a, const mgp_array* b, void* stream) {
+    if (!valid_array(a) || !valid_array(b) || !same_shape(a, b)) return MGP_ERR_ARG;
+    const Box bx = shape_box(a);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL(k_add<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*a), view_of<T>(*b), bx,
+                       g.zchunks);
+    return launch_status();
+}
+
+template <class T>
+static int norm_inf(const mgp_array* a, double* out, void* stream) {
+    if (!valid_array(a) || !out) return MGP_ERR_ARG;
+    const Box bx = shape_box(a);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL(k_norm<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*a), bx, g.zchunks,
+                       (unsigned long long*)out);
+    return launch_status();
+}
+
+// constants exactly as the reference forms them from T omega and double h
+// (include/gaussseidel.h:50-52, jacobi.h:42-44)
+template <class T>
+struct RelaxConsts {
+    T one_omega, omega_sixth, hsq;
+    RelaxConsts(double omega_in, double h) {
+        const T omega = (T)omega_in;
+        one_omega = (T)(1.0 - omega);
+        omega_sixth = (T)(omega / 6.0);
+        hsq = (T)(h * h);
+    }
+};
+
+template <class T>
+static int rbgs_half(int mode, const mgp_array* u, const mgp_array* f, const int64_t b[6], double omega, double h,
+                     int colour, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !same_shape(u, f) || !valid_box(u, b) || (colour & ~1)) return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    if (box_empty(bx)) return MGP_OK;
+    const BoxGrid g = box_grid(bx);
+    const RelaxConsts<T> c(omega, h);
+    if (mode == MGP_MODE_REFERENCE)
+        hipLaunchKernelGGL((k_gs_half<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*u),
+                           view_of<T>(*f), bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, colour, lin0_of(u));
+    else if (mode == MGP_MODE_CONSISTENT)
+        hipLaunchKernelGGL((k_gs_half<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, (hipStream_t)stream,
+                           view_of<T>(*u), view_of<T>(*f), bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, colour,
+                           lin0_of(u));
+    else
+        return MGP_ERR_ARG;
+    return launch_status();
+}
+
+static inline bool checkerboard(int mode, const mgp_array* u) {
+    return mode == MGP_MODE_CONSISTENT || ((u->stride[0] & 1) && (u->stride[1] & 1));
+}
+
+template <class T>
+static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                      const int64_t b[6], double omega, void* stream) {
+    if (!valid_array(ui) || !valid_array(uo) || !valid_array(f) || !faces || !same_shape(ui, uo) ||
+        !same_shape(ui, f) || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    if (!checkerboard(mode, ui) || lin0_of(ui) != lin0_of(uo) || ui->stride[0] != uo->stride[0] ||
+        ui->stride[1] != uo->stride[1])
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < 0 || b[2 * d + 1] > ui->shape[d]) return MGP_ERR_ARG;
+    // a box point on a face with a Dirichlet type would need a ghost beyond it
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == ui->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == ui->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == ui->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    if (box_empty(bx)) return MGP_OK;
+    const BoxGrid g = box_grid(bx);
+    const RelaxConsts<T> c(omega, faces->h);
+    const Faces<T> F = faces_of<T>(*faces);
+    const hipStream_t s = (hipStream_t)stream;
+    if (mode == MGP_MODE_REFERENCE) {
+        hipLaunchKernelGGL((k_sweep_red<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*ui),
+                           view_of<T>(*uo), view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq,
+                           lin0_of(ui));
+        hipLaunchKernelGGL((k_sweep_black<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*uo),
+                           view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, lin0_of(uo));
+    } else {
+        hipLaunchKernelGGL((k_sweep_red<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*ui),
+                           view_of<T>(*uo), view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq,
+                           lin0_of(ui));
+        hipLaunchKernelGGL((k_sweep_black<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*uo),
+                           view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, lin0_of(uo));
+    }
+    return launch_status();
+}
+
+template <class T>
+static int jacobi(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6],
+                  double omega, double h, void* stream) {
+    if (!valid_array(u) || !valid_array(v) || !valid_array(f) || !same_shape(u, v) || !same_shape(u, f) ||
+        !valid_box(v, b))
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    if (box_empty(bx)) return MGP_OK;
+    const BoxGrid g = box_grid(bx);
+    const RelaxConsts<T> c(omega, h);
+    if (mode == MGP_MODE_REFERENCE)
+        hipLaunchKernelGGL((k_jacobi<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*u),
+                           view_of<T>(*v), view_of<T>(*f), bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq);
+    else if (mode == MGP_MODE_CONSISTENT)
+        hipLaunchKernelGGL((k_jacobi<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*u),
+                           view_of<T>(*v), view_of<T>(*f), bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq);
+    else
+        return MGP_ERR_ARG;
+    return launch_status();
+}
+
+template <class T>
+static int residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6], double h,
+                    double* norm, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !same_shape(u, f) || !same_shape(u, r) ||
+        !valid_box(u, b) || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    const T hsq = (T)(h * h);  // include/residual.h:23
+    const T inv_h2 = (T)(1.0 / (h * h));
+    const hipStream_t s = (hipStream_t)stream;
+    if (norm) {
+        const Box sh = shape_box(r);
+        const BoxGrid g = box_grid(sh);
+        if (mode == MGP_MODE_REFERENCE)
+            hipLaunchKernelGGL((k_residual_norm<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*u),
+                               view_of<T>(*f), view_of<T>(*r), sh, g.zchunks, bx, hsq, inv_h2,
+                               (unsigned long long*)norm);
+        else
+            hipLaunchKernelGGL((k_residual_norm<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*u),
+                               view_of<T>(*f), view_of<T>(*r), sh, g.zchunks, bx, hsq, inv_h2,
+                               (unsigned long long*)norm);
+    } else {
+        if (box_empty(bx)) return MGP_OK;
+        const BoxGrid g = box_grid(bx);
+        if (mode == MGP_MODE_REFERENCE)
+            hipLaunchKernelGGL((k_residual<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*u),
+                               view_of<T>(*f), view_of<T>(*r), bx, g.zchunks, hsq, inv_h2);
+        else
+            hipLaunchKernelGGL((k_residual<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*u),
+                               view_of<T>(*f), view_of<T>(*r), bx, g.zchunks, hsq, inv_h2);
+    }
+    return launch_status();
+}
+
+static inline bool coarse_of(const mgp_array* fine, const mgp_array* coarse) {
+    for (int d = 0; d < 3; d++)
+        if ((fine->shape[d] - 1) / 2 + 1 != coarse->shape[d]) return false;
+    return true;
+}
+
+template <class T>
+static int restrict_any(bool fw, const mgp_array* in, const mgp_array* out, void* stream) {
+    if (!valid_array(in) || !valid_array(out) || !coarse_of(in, out)) return MGP_ERR_ARG;
+    const Box bx = shape_box(out);
+    const BoxGrid g = box_grid(bx);
+    if (fw)
+        hipLaunchKernelGGL(k_fw<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*in), view_of<T>(*out), bx,
+                           g.zchunks);
+    else
+        hipLaunchKernelGGL(k_inject<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*in), view_of<T>(*out), bx,
+                           g.zchunks);
+    return launch_status();
+}
+
+template <class T, bool ADD>
+static int prolong(const mgp_array* coarse, const mgp_array* fine, void* stream) {
+    if (!valid_array(coarse) || !valid_array(fine)) return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)  // exact 2:1 nesting (trilinearinterpolation.h:31-37)
+        if (2 * (coarse->shape[d] - 1) + 1 != fine->shape[d]) return MGP_ERR_ARG;
+    const Box bx = shape_box(fine);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL((k_prolong<T, ADD>), g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*coarse),
+                       view_of<T>(*fine), bx, g.zchunks);
+    return launch_status();
+}
+
+static inline int64_t face_points(int loc, const mgp_array* u) {
+    int64_t n = 1;
+    for (int d = 0; d < 3; d++) {
+        const bool flat = (d == 0 && (loc == MGP_EAST || loc == MGP_WEST)) ||
+                          (d == 1 && (loc == MGP_NORTH || loc == MGP_SOUTH)) ||
+                          (d == 2 && (loc == MGP_TOP || loc == MGP_BOTTOM));
+        n *= flat ? 1 : u->shape[d];
+    }
+    return n;
+}
+
+static inline int64_t halo_of_face(const mgp_array* u, int loc) { return u->halo[loc]; }
+
+template <class T>
+static int bc_dirichlet(const mgp_array* u, int loc, const void* g, void* stream) {
+    if (!valid_array(u) || loc < 0 || loc > 5 || !g) return MGP_ERR_ARG;
+    const int64_t n = face_points(loc, u);
+    hipLaunchKernelGGL(k_bc_dirichlet<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       view_of<T>(*u), loc, (const T*)g);
+    return launch_status();
+}
+
+template <class T>
+static int bc_neumann(const mgp_array* u, int loc, const void* g, double h, void* stream) {
+    if (!valid_array(u) || loc < 0 || loc > 5 || halo_of_face(u, loc) < 1) return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+        if (u->shape[d] < 3) return MGP_ERR_ARG;  // the mirror point must exist
+    const int64_t n = face_points(loc, u);
+    const T two_h = (T)(2.0 * h);  // include/neumann.h:108
+    hipLaunchKernelGGL(k_bc_neumann<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       view_of<T>(*u), loc, (const T*)g, two_h);
+    return launch_status();
+}
+
+static inline mgp_array face_array(void* data, int loc, const int64_t n[3]) {
+    mgp_array a{};
+    a.data = data;
+    for (int d = 0; d < 3; d++) a.shape[d] = n[d];
+    if (loc == MGP_EAST || loc == MGP_WEST) a.shape[0] = 1;
+    if (loc == MGP_NORTH || loc == MGP_SOUTH) a.shape[1] = 1;
+    if (loc == MGP_TOP || loc == MGP_BOTTOM) a.shape[2] = 1;
+    a.stride[2] = 1;
+    a.stride[1] = a.shape[2];
+    a.stride[0] = a.shape[1] * a.shape[2];
+    return a;
+}
+
+template <class T>
+static int bc_restrict_neumann(int restriction, const mgp_array* uf, int loc, const void* gf, void* gc, double h_fine,
+                               void* stream) {
+    if (!valid_array(uf) || loc < 0 || loc > 5 || !gf || !gc ||
+        (restriction != MGP_INJECTION && restriction != MGP_FULL_WEIGHTING))
+        return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+        if (uf->shape[d] < 3 || (uf->shape[d] & 1) == 0) return MGP_ERR_ARG;
+    const int64_t nc[3] = {uf->shape[0] / 2 + 1, uf->shape[1] / 2 + 1, uf->shape[2] / 2 + 1};
+    const mgp_array F = face_array((void*)gf, loc, uf->shape);
+    const mgp_array C = face_array(gc, loc, nc);
+    // neumann.h:198-211
+    T c1 = -3.0 / 2.0, c2 = 2.0, c3 = -0.5;
+    if (loc == MGP_EAST || loc == MGP_NORTH || loc == MGP_TOP) {
+        c3 = 0.5;
+        c2 = -2.0;
+        c1 = 3.0 / 2.0;
+    }
+    const T hf = (T)h_fine;  // neumann.h:221
+    const int64_t n = C.shape[0] * C.shape[1] * C.shape[2];
+    hipLaunchKernelGGL(k_bc_restrict_neumann<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)stream,
+                       restriction, view_of<T>(*uf), loc, view_of<T>(F), view_of<T>(C), c1, c2, c3, hf);
+    return launch_status();
+}
+
+}  // namespace mgp
+
+// ============================================================ C entry points
+using namespace mgp;
+
+extern "C" {
+
+int mgp_fill_f64(const mgp_array* a, double v, void* s) { return fill<double>(a, v, s); }
+int mgp_fill_f32(const mgp_array* a, double v, void* s) { return fill<float>(a, v, s); }
+int mgp_add_f64(const mgp_array* a, const mgp_array* b, void* s) { return add<double>(a, b, s); }
+int mgp_add_f32(const mgp_array* a, const mgp_array* b, void* s) { return add<float>(a, b, s); }
+int mgp_norm_inf_f64(const mgp_array* a, double* o, void* s) { return norm_inf<double>(a, o, s); }
+int mgp_norm_inf_f32(const mgp_array* a, double* o, void* s) { return norm_inf<float>(a, o, s); }
+
+int mgp_zero_scalars(double* p, int64_t n, void* s) {
+    if (!p || n < 0) return MGP_ERR_ARG;
+    if (n == 0) return MGP_OK;
+    hipLaunchKernelGGL(k_zero_scalars, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, (hipStream_t)s, p, n);
+    return launch_status();
+}
+
+int mgp_rbgs_half_f64(int m, const mgp_array* u, const mgp_array* f, const int64_t b[6], double w, double h, int c,
+                      void* s) {
+    return rbgs_half<double>(m, u, f, b, w, h, c, s);
+}
+int mgp_rbgs_half_f32(int m, const mgp_array* u, const mgp_array* f, const int64_t b[6], double w, double h, int c,
+                      void* s) {
+    return rbgs_half<float>(m, u, f, b, w, h, c, s);
+}
+int mgp_rbgs_sweep_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                       const int64_t b[6], double w, void* s) {
+    return rbgs_sweep<double>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                       const int64_t b[6], double w, void* s) {
+    return rbgs_sweep<float>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_jacobi_f64(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], double w,
+                   double h, void* s) {
+    return jacobi<double>(m, u, v, f, b, w, h, s);
+}
+int mgp_jacobi_f32(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], double w,
+                   double h, void* s) {
+    return jacobi<float>(m, u, v, f, b, w, h, s);
+}
+int mgp_residual_f64(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6], double h,
+                     double* n, void* s) {
+    return residual<double>(m, u, f, r, b, h, n, s);
+}
+int mgp_residual_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6], double h,
+                     double* n, void* s) {
+    return residual<float>(m, u, f, r, b, h, n, s);
+}
+int mgp_restrict_inject_f64(const mgp_array* i, const mgp_array* o, void* s) { return restrict_any<double>(false, i, o, s); }
+int mgp_restrict_inject_f32(const mgp_array* i, const mgp_array* o, void* s) { return restrict_any<float>(false, i, o, s); }
+int mgp_restrict_fw_f64(const mgp_array* i, const mgp_array* o, void* s) { return restrict_any<double>(true, i, o, s); }
+int mgp_restrict_fw_f32(const mgp_array* i, const mgp_array* o, void* s) { return restrict_any<float>(true, i, o, s); }
+int mgp_prolong_trilinear_f64(const mgp_array* c, const mgp_array* f, void* s) { return prolong<double, false>(c, f, s); }
+int mgp_prolong_trilinear_f32(const mgp_array* c, const mgp_array* f, void* s) { return prolong<float, false>(c, f, s); }
+int mgp_prolong_add_f64(const mgp_array* c, const mgp_array* u, void* s) { return prolong<double, true>(c, u, s); }
+int mgp_prolong_add_f32(const mgp_array* c, const mgp_array* u, void* s) { return prolong<float, true>(c, u, s); }
+int mgp_bc_dirichlet_f64(const mgp_array* u, int l, const void* g, void* s) { return bc_dirichlet<double>(u, l, g, s); }
+int mgp_bc_dirichlet_f32(const mgp_array* u, int l, const void* g, void* s) { return bc_dirichlet<float>(u, l, g, s); }
+int mgp_bc_neumann_f64(const mgp_array* u, int l, const void* g, double h, void* s) {
+    return bc_neumann<double>(u, l, g, h, s);
+}
+int mgp_bc_neumann_f32(const mgp_array* u, int l, const void* g, double h, void* s) {
+    return bc_neumann<float>(u, l, g, h, s);
+}
+int mgp_bc_restrict_neumann_f64(int r, const mgp_array* u, int l, const void* gf, void* gc, double h, void* s) {
+    return bc_restrict_neumann<double>(r, u, l, gf, gc, h, s);
+}
+int mgp_bc_restrict_neumann_f32(int r, const mgp_array* u, int l, const void* gf, void* gc, double h, void* s) {
+    return bc_restrict_neumann<float>(r, u, l, gf, gc, h, s);
+}
+
+}  // extern "C"

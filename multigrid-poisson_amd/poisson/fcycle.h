@@ -1,0 +1,21 @@
+// F-cycle (reference include/fcycle.h:13-97): pre-smooth, descend, F on the
+// next level (while two or more levels remain below), then V on the next
+// level, prolong and correct, and a closing V-cycle from the finest level.
+#ifndef POISSON_FCYCLE
+#define POISSON_FCYCLE
+
+#include "vcycle.h"
+
+namespace Poisson {
+template <class T>
+void Fcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& prolongation,
+            Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10) {
+    relaxation.relax(domains, omega, nsmooth, level, num_devices);
+    detail::descend(domains, restriction, level, num_devices);
+    if (level < levels - 2) Fcycle(domains, restriction, prolongation, relaxation, omega, level + 1, levels, num_devices, nsmooth);
+    Vcycle<T>(domains, restriction, prolongation, relaxation, omega, level + 1, levels, num_devices, nsmooth);
+    detail::ascend(domains, prolongation, level, num_devices);
+    if (level == 0) Vcycle<T>(domains, restriction, prolongation, relaxation, omega, 0, levels, num_devices, nsmooth);
+}
+}  // namespace Poisson
+#endif

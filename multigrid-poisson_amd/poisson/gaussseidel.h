@@ -1,0 +1,62 @@
+// Red-black Gauss-Seidel SOR, default omega 8/9, one colour per
+// relaxation_kernel call starting with red (reference include/gaussseidel.h:14-99).
+//
+// Where the colouring is a checkerboard (always in CONSISTENT mode; in
+// REFERENCE mode when both outer strides are odd) and Settings::fused_sweeps is
+// on, relax_level runs each red+black pair as one out-of-place pass
+// (mgp_rbgs_sweep) that forms Neumann ghosts on the fly; the values are
+// identical to the two half-sweeps with boundary refreshes in between.
+#ifndef POISSON_GAUSS_SEIDEL
+#define POISSON_GAUSS_SEIDEL
+
+#include "relaxation.h"
+
+namespace Poisson {
+template <class T>
+class GaussSeidel : public Relaxation<T> {
+private:
+    bool alternate = true;
+
+public:
+    T default_omega() override { return 8.0 / 9.0; }
+    int smoothing_multiplier() override { return 2; }
+    bool requires_duplicate_solution() override { return false; }
+
+    void relaxation_kernel(Domain<T>& domain, T omega, const int_t xmin, const int_t xmax, const int_t ymin,
+                           const int_t ymax, const int_t zmin, const int_t zmax) override {
+        const int64_t b[6] = {xmin, xmax, ymin, ymax, zmin, zmax};
+        const mgp_array u = domain.u->desc(), f = domain.f->desc();
+        mgpx::check(mgpx::Ops<T>::rbgs_half(domain.settings.mode, &u, &f, b, (double)omega, domain.settings.h,
+                                            alternate ? 0 : 1, mgpx::stream()),
+                    "GaussSeidel::relaxation_kernel");
+        alternate = !alternate;
+    }
+
+    static bool checkerboard(const Domain<T>& d) {
+        return d.settings.mode == CONSISTENT || ((d.u->stride[0] & 1) && (d.u->stride[1] & 1));
+    }
+
+    void relax_level(Domain<T>& d, T omega, int_t nsmooth) override {
+        if (!d.u_alt || !d.settings.fused_sweeps || !checkerboard(d) || !alternate || nsmooth < 1) {
+            Relaxation<T>::relax_level(d, omega, nsmooth);
+            return;
+        }
+        int64_t b[6];
+        d.box(b);
+        const mgp_faces F = d.faces();
+        const mgp_array f = d.f->desc();
+        for (int_t s = 0; s < nsmooth; s++) {
+            mgp_array ui = d.u->desc();
+            mgp_array uo = ui;
+            uo.data = d.u_alt;
+            mgpx::check(mgpx::Ops<T>::rbgs_sweep(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega, mgpx::stream()),
+                        "GaussSeidel sweep");
+            d.swap_u_alt();
+        }
+    }
+
+    int mgp_state() const override { return alternate ? 1 : 0; }
+    void mgp_set_state(int s) override { alternate = s != 0; }
+};
+}  // namespace Poisson
+#endif

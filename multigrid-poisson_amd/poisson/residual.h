@@ -1,0 +1,43 @@
+// Residual r = f - L_h u over the relaxation box after the boundary refresh
+// (reference include/residual.h:18-81).
+#ifndef POISSON_RESIDUAL
+#define POISSON_RESIDUAL
+
+#include "domain.h"
+
+namespace Poisson {
+namespace detail {
+template <class T>
+inline void refresh_boundaries(Domain<T>& domain, bool fetch_neighbor) {
+    domain.east->update(domain, false, fetch_neighbor);
+    domain.west->update(domain, false, fetch_neighbor);
+    domain.north->update(domain, false, fetch_neighbor);
+    domain.south->update(domain, false, fetch_neighbor);
+    domain.top->update(domain, false, fetch_neighbor);
+    domain.bottom->update(domain, false, fetch_neighbor);
+}
+}  // namespace detail
+
+template <class T>
+void residual(Domain<T>& domain, bool fetch_neighbor) {
+    detail::refresh_boundaries(domain, fetch_neighbor);
+    int64_t b[6];
+    domain.box(b);
+    const mgp_array u = domain.u->desc(), f = domain.f->desc(), r = domain.r->desc();
+    mgpx::check(mgpx::Ops<T>::residual(domain.settings.mode, &u, &f, &r, b, domain.settings.h, nullptr, mgpx::stream()),
+                "residual");
+}
+
+// addition: residual plus max |r| over r's shape into *norm_dev (a device
+// double the caller zeroed), in one pass
+template <class T>
+void residual_norm(Domain<T>& domain, double* norm_dev, bool fetch_neighbor = true) {
+    detail::refresh_boundaries(domain, fetch_neighbor);
+    int64_t b[6];
+    domain.box(b);
+    const mgp_array u = domain.u->desc(), f = domain.f->desc(), r = domain.r->desc();
+    mgpx::check(mgpx::Ops<T>::residual(domain.settings.mode, &u, &f, &r, b, domain.settings.h, norm_dev, mgpx::stream()),
+                "residual");
+}
+}  // namespace Poisson
+#endif

@@ -1,0 +1,54 @@
+// Recursive V-cycle (reference include/vcycle.h:14-98): pre-smooth; on the
+// coarsest level smoothing only; defect, restriction to the coarse right-hand
+// side, zero coarse guess, coarse boundary data, recursion, prolongation and
+// correction, post-smooth.
+#ifndef POISSON_VCYCLE
+#define POISSON_VCYCLE
+
+#include "gaussseidel.h"
+#include "jacobi.h"
+#include "residual.h"
+
+namespace Poisson {
+namespace detail {
+// vcycle.h:34-76 (also fcycle.h:130-172)
+template <class T>
+void descend(Domain<T>*** domains, Restriction<T>& restriction, uint_t level, uint_t num_devices) {
+    for (uint_t g = 0; g < num_devices; g++) {
+        Domain<T>& d = *domains[g][level];
+        Domain<T>& c = *domains[g][level + 1];
+        residual<T>(d);
+        restriction.restrict_to(*d.r, *c.f);
+        c.u->init_zero();
+        c.north->restrict_to(*d.u, *d.north, d.settings, restriction);
+        c.south->restrict_to(*d.u, *d.south, d.settings, restriction);
+        c.east->restrict_to(*d.u, *d.east, d.settings, restriction);
+        c.west->restrict_to(*d.u, *d.west, d.settings, restriction);
+        c.top->restrict_to(*d.u, *d.top, d.settings, restriction);
+        c.bottom->restrict_to(*d.u, *d.bottom, d.settings, restriction);
+    }
+}
+
+// vcycle.h:81-94: u_l += P(u_{l+1}) with r_l as the scratch
+template <class T>
+void ascend(Domain<T>*** domains, Prolongation<T>& prolongation, uint_t level, uint_t num_devices) {
+    for (uint_t g = 0; g < num_devices; g++) {
+        Domain<T>& d = *domains[g][level];
+        Domain<T>& c = *domains[g][level + 1];
+        prolongation.prolong_add(*c.u, *d.u, *d.r);
+    }
+}
+}  // namespace detail
+
+template <class T>
+void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& prolongation,
+            Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10) {
+    relaxation.relax(domains, omega, nsmooth, level, num_devices);
+    if (level >= levels - 1) return;
+    detail::descend(domains, restriction, level, num_devices);
+    Vcycle<T>(domains, restriction, prolongation, relaxation, omega, level + 1, levels, num_devices, nsmooth);
+    detail::ascend(domains, prolongation, level, num_devices);
+    relaxation.relax(domains, omega, nsmooth, level, num_devices);
+}
+}  // namespace Poisson
+#endif
