@@ -1,0 +1,198 @@
+#!/usr/bin/env python3
+"""Throughput of the multigrid solve on one MI355X.
+
+Workload (the headline configuration): Poisson on a 1025 x 1025 x 513 grid
+(h = 1/1024, x, y in [0,1], z in [0,0.5]), 7 levels, F-cycles with red-black
+Gauss-Seidel (omega = 1, 2 sweeps), full-weighting restriction, trilinear
+prolongation, Neumann faces in x and y, Dirichlet in z, u = x^3 y^2 z,
+f = 2x^3 z + 6x y^2 z, FP64, solved from u = 0 to ||r||inf/||f||inf <= 1e-10
+(consistent boundary treatment: the reference's own coarse Neumann data stalls
+at O(h) and never reaches that tolerance).
+
+A step is one complete solve to the tolerance; the right-hand side and the
+hierarchy stay resident in HBM, each step restarts from u = 0 on the device.
+value = fine-grid DOF x steps / wall time of the timed steps.
+
+Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
+runs on one GPU.
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "multigrid-poisson_amd"))
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import mgp  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, published
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def kernel_time_ms(fn, reps, stream):
+    """Average duration of `fn(stream)` over `reps` launches, HIP events on the
+    stream the kernel is launched on."""
+    L = mgp.lib()
+    e0, e1 = mgp.vp(), mgp.vp()
+    mgp.check(L.mgp_event_create(ctypes.byref(e0)))
+    mgp.check(L.mgp_event_create(ctypes.byref(e1)))
+    fn(stream)  # warm
+    mgp.check(L.mgp_event_record(e0, stream))
+    for _ in range(reps):
+        fn(stream)
+    mgp.check(L.mgp_event_record(e1, stream))
+    mgp.check(L.mgp_event_sync(e1))
+    ms = ctypes.c_float()
+    mgp.check(L.mgp_event_elapsed_ms(e0, e1, ctypes.byref(ms)))
+    L.mgp_event_destroy(e0)
+    L.mgp_event_destroy(e1)
+    return ms.value / reps
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=1025, help="points along x and y (z gets n//2+1)")
+    ap.add_argument("--levels", type=int, default=7)
+    ap.add_argument("--tol", type=float, default=1e-10)
+    ap.add_argument("--nsmooth", type=int, default=2)
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs the solve on one GPU; --gpus %d is not supported "
+              "(multi-GPU sharding is a design note only)" % args.gpus, file=sys.stderr)
+        sys.exit(2)
+
+    mgp.require_gpu()
+    n, nz = args.n, args.n // 2 + 1
+    dims = (n, n, nz)
+    h = 1.0 / (n - 1)
+    dof = n * n * nz
+    log("bench: %dx%dx%d, %d levels, F-cycle GS(%d) FW, tol %.0e" % (n, n, nz, args.levels, args.nsmooth, args.tol))
+    s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET, maxiter=100, miniter=1, tolerance=args.tol)
+    s.to_device()
+    log("bench: hierarchy resident; warmup")
+    for _ in range(args.warmup):
+        s.reset()
+        s.solve("Fcycle", args.nsmooth, 1.0)
+    s.sync()
+    cycles, rel = [], []
+    t0 = time.perf_counter()
+    for k in range(args.steps):
+        s.reset()
+        it, rr, _t = s.solve("Fcycle", args.nsmooth, 1.0)
+        cycles.append(it + 1)
+        rel.append(rr)
+    s.sync()
+    elapsed = time.perf_counter() - t0
+    ms_per_step = 1e3 * elapsed / args.steps
+    value = dof * args.steps / elapsed
+    log("bench: %.1f ms/solve, %d cycles, rel_res %.3e" % (ms_per_step, cycles[-1], rel[-1]))
+    if max(rel) > args.tol:
+        log("bench: WARNING: tolerance not reached")
+
+    # Dominant kernels measured live on the fine level: the fused red-black
+    # sweep (24 B per box point: read u and f, write u) and the residual with
+    # its max-norm (24 B per point: read u and f, write r).
+    L = mgp.lib()
+    stream = mgp.vp()
+    mgp.check(L.mgp_stream_create(ctypes.byref(stream)))
+    u = s.device_array("u")
+    f_ = s.device_array("f")
+    r = s.device_array("r")
+    F = mgp.Faces()
+    for q in range(6):
+        t = s.face_type(q)
+        F.type[q] = t
+        host = mgp.vp()
+        a = mgp.Array()
+        tt = ctypes.c_int()
+        mgp.check(L.mgp_solver_face(s.h, 0, q, ctypes.byref(host), ctypes.byref(a), ctypes.byref(tt)))
+        F.g[q] = a.data if t == mgp.NEUMANN else None
+    F.h = h
+    box = [0, n, 0, n, 1, nz - 1]
+    box_pts = n * n * (nz - 2)
+    bx = mgp.box(box)
+    scal = mgp.DeviceBuffer(__import__("numpy").zeros(1))
+    # u -> r and back keeps every access inside the level's arrays; the values
+    # are not used afterwards
+    sweep_ms = kernel_time_ms(
+        lambda st: mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_),
+                                                   ctypes.byref(F), bx, 1.0, st)), args.kernel_reps, stream)
+    resid_ms = kernel_time_ms(
+        lambda st: mgp.check(L.mgp_residual_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(f_), ctypes.byref(r),
+                                                 bx, h, scal.ptr, st)), args.kernel_reps, stream)
+    L.mgp_stream_sync(stream)
+    sweep_bytes = 24.0 * box_pts
+    resid_bytes = 24.0 * box_pts
+    sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    resid_gbs = resid_bytes / (resid_ms * 1e-3) / 1e9
+    log("bench: sweep %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" % (sweep_ms, sweep_gbs, resid_ms, resid_gbs))
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads)
+
+    out = {
+        "metric": "DOF/s to ||r||/||f|| <= 1e-10 (plain FP64 F-cycle, 1025x1025x513)",
+        "value": value,
+        "unit": "DOF/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (analytic u = x^3 y^2 z)",
+        "config": {"workload": "poisson-fcycle-%dx%dx%d" % dims, "grid": list(dims), "levels": args.levels,
+                   "cycle": "F", "smoother": "red-black GS omega=1, %d sweeps" % args.nsmooth,
+                   "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
+                   "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "rbgs_sweep (fine level)", "achieved": sweep_gbs,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
+        "kernels": {"residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+def cpu_baseline(dims, levels, h, nsmooth, cycles, threads):
+    """The oracle (this build's C restatement, OpenMP) on the host cores: one
+    F-cycle plus the residual check at the full grid, scaled to the GPU's
+    cycle count to the tolerance."""
+    import oracle_lib as O
+
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0))
+    log("bench: CPU baseline (oracle, %d threads): setup" % threads)
+    o = O.Solver(dims, levels, h, (0.0, 0.0, 0.0), "NNNNDD", O.CONSISTENT, O.GAUSS_SEIDEL, O.FULL_WEIGHTING, 2,
+                 threads=threads)
+    o.to_device()
+    t0 = time.perf_counter()
+    o.solve(O.FCYCLE, nsmooth, 1.0, maxiter=1, miniter=1, tol=0.0)
+    t = time.perf_counter() - t0
+    o.close()
+    dof = dims[0] * dims[1] * dims[2]
+    log("bench: CPU baseline %.2f s per F-cycle" % t)
+    return {"value": dof / (t * cycles), "unit": "DOF/s", "cores": threads, "kind": "port",
+            "sample": "1 F-cycle + residual at %dx%dx%d (%.1f s), scaled to %d cycles" % (dims + (t, cycles)),
+            "s_per_cycle": t}
+
+
+if __name__ == "__main__":
+    main()

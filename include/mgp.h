@@ -223,6 +223,9 @@ int mgp_solver_solve(void* solver, const char* cycle, int64_t nsmooth, double om
 int mgp_solver_cycle(void* solver, const char* cycle, int64_t nsmooth, double omega);
 int mgp_solver_relres(void* solver, double* relres);
 int mgp_solver_sync(void* solver);
+/* every level's u back to zero with its boundary data, on the device, for a
+ * fresh solve with unchanged right-hand side and faces (no host transfer) */
+int mgp_solver_reset(void* solver);
 /* relative_residual(), solve_time(), solve_iterations() (libpoisson.h:378-393) */
 int mgp_solver_stats(void* solver, double* rel_res, double* seconds, int64_t* iterations);
 /* host mirror and device descriptor of level `level`'s u ('u'), uprev ('p'),

@@ -24,9 +24,17 @@ inline double f(double x, double y, double z) {
     case 0: { const double p = 2.0 * M_PI; return -3.0 * p * p * sin(p * x) * sin(p * y) * sin(p * z); }
     case 1: return -(3.0 * 3.0 + 5.3 * 5.3 + -7.8 * -7.8) * sin(3.0 * x) * sin(5.3 * y) * sin(-7.8 * z);
     case 2: return 2.0 * x * x * x * z + 6.0 * x * y * y * z;
-    default:
-        return ((-4.0 * x * x * z * z - 9.0 * y * y * y * y - z * z * z * z) * sin(y * y * y) +
-                6.0 * y * cos(y * y * y)) * cos(x * z * z) - 2.0 * x * sin(x * z * z) * sin(y * y * y);
+    default: {
+        // sin and cos of one argument through sincos(): GCC merges such pairs
+        // into sincos (whose last bit can differ from sin's) and the reference
+        // was built with GCC, so the explicit call keeps every compiler on the
+        // reference's values
+        double sy, cy, sxz, cxz;
+        sincos(y * y * y, &sy, &cy);
+        sincos(x * z * z, &sxz, &cxz);
+        return ((-4.0 * x * x * z * z - 9.0 * y * y * y * y - z * z * z * z) * sy + 6.0 * y * cy) * cxz -
+               2.0 * x * sxz * sy;
+    }
     }
 }
 inline double dudx(double x, double y, double z) {

@@ -24,6 +24,7 @@ struct SolverBase {
     virtual void cycle(const std::string& c, int64_t nsmooth, double omega) = 0;
     virtual double relres() = 0;
     virtual void sync() = 0;
+    virtual void reset() = 0;
     virtual void stats(double* rr, double* t, int64_t* it) = 0;
     virtual int array(int level, int which, void** host, mgp_array* dev) = 0;
     virtual int face(int level, int loc, void** host, mgp_array* dev, int* type) = 0;
@@ -47,6 +48,7 @@ struct SolverImpl : SolverBase {
     void cycle(const std::string& c, int64_t nsmooth, double omega) override { s.cycle(c, nsmooth, (T)omega); }
     double relres() override { return s.relres(); }
     void sync() override { Poisson::mgpx::check(mgp_stream_sync(s.get_stream()), "sync"); }
+    void reset() override { s.reset_solution(); }
     void stats(double* rr, double* t, int64_t* it) override {
         if (rr) *rr = (double)s.relative_residual();
         if (t) *t = (double)s.solve_time();
@@ -183,6 +185,7 @@ int mgp_solver_relres(void* h, double* rr) {
 }
 
 int mgp_solver_sync(void* h) { return h ? guarded([&] { S(h)->sync(); }) : MGP_ERR_ARG; }
+int mgp_solver_reset(void* h) { return h ? guarded([&] { S(h)->reset(); }) : MGP_ERR_ARG; }
 
 int mgp_solver_stats(void* h, double* rr, double* t, int64_t* it) {
     if (!h) return MGP_ERR_ARG;

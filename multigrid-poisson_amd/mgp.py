@@ -62,7 +62,7 @@ _SIGS = {
     "mgp_solver_create": [P(SolverConfig), P(vp)], "mgp_solver_destroy": [vp],
     "mgp_solver_init_problem": [vp, i32], "mgp_solver_init_zero": [vp], "mgp_solver_to_device": [vp],
     "mgp_solver_to_host": [vp], "mgp_solver_solve": [vp, ctypes.c_char_p, i64, d, d],
-    "mgp_solver_cycle": [vp, ctypes.c_char_p, i64, d], "mgp_solver_relres": [vp, P(d)], "mgp_solver_sync": [vp],
+    "mgp_solver_cycle": [vp, ctypes.c_char_p, i64, d], "mgp_solver_relres": [vp, P(d)], "mgp_solver_sync": [vp], "mgp_solver_reset": [vp],
     "mgp_solver_stats": [vp, P(d), P(d), P(i64)], "mgp_solver_array": [vp, i32, i32, P(vp), P(Array)],
     "mgp_solver_face": [vp, i32, i32, P(vp), P(Array), P(ctypes.c_int)], "mgp_solver_save": [vp, ctypes.c_char_p],
 }
@@ -314,6 +314,9 @@ class Solver:
 
     def sync(self):
         check(lib().mgp_solver_sync(self.h), "sync")
+
+    def reset(self):
+        check(lib().mgp_solver_reset(self.h), "reset")
 
     def stats(self):
         rr, t, it = ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()

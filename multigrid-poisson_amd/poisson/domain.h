@@ -101,6 +101,19 @@ public:
         return F;
     }
     void swap_u_alt() { std::swap(u->devptr, u_alt); }
+    // u (and uprev) = 0 with the boundary data written again, on the device;
+    // the state to_device() leaves for a fresh solve with unchanged f and faces
+    void reset_solution() {
+        u->init_zero();
+        write_bc_to(*u);
+        if (requires_duplicate) {
+            uprev->init_zero();
+            write_bc_to(*uprev);
+        }
+        if (u_alt)
+            mgpx::check(mgp_memcpy_d2d(u_alt, u->devptr, (size_t)u->size * sizeof(T), mgpx::stream()),
+                        "Domain::reset_solution");
+    }
 };
 
 template <class T>

@@ -125,6 +125,12 @@ public:
     void cycle(const std::string solver, const int_t nsmooth = 10, T omega = -1.0);
     // residual on level 0 and ||r||inf / ||f||inf (waits)
     double relres();
+    // every level's u back to zero with its boundary data, on the device (waits)
+    void reset_solution() {
+        mgpx::StreamScope scope(stream);
+        for (uint_t l = 0; l < settings.levels; l++) domains[0][l]->reset_solution();
+        mgpx::check(mgp_stream_sync(stream), "reset_solution");
+    }
 };
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>
