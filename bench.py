@@ -65,8 +65,13 @@ def main():
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--nsmooth", type=int, default=2)
+    ap.add_argument("--coarse-sweeps", type=int, default=32,
+                    help="smoothing passes per visit of the coarsest level (17x17x9)")
     ap.add_argument("--kernel-reps", type=int, default=10)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (for profilers)")
+    ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
+                    help="mgp_tuning_set before the run (e.g. sweep.target_wgs=4096)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -75,13 +80,18 @@ def main():
         sys.exit(2)
 
     mgp.require_gpu()
+    for kv in args.tune:
+        k, v = kv.split("=")
+        mgp.check(mgp.lib().mgp_tuning_set(k.encode(), int(v)), "mgp_tuning_set")
     n, nz = args.n, args.n // 2 + 1
     dims = (n, n, nz)
     h = 1.0 / (n - 1)
     dof = n * n * nz
     log("bench: %dx%dx%d, %d levels, F-cycle GS(%d) FW, tol %.0e" % (n, n, nz, args.levels, args.nsmooth, args.tol))
     s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
-                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET, maxiter=100, miniter=1, tolerance=args.tol)
+                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0),
+                   maxiter=100, miniter=1, tolerance=args.tol,
+                   coarse_sweeps=args.coarse_sweeps)
     s.to_device()
     log("bench: hierarchy resident; warmup")
     for _ in range(args.warmup):
@@ -143,7 +153,7 @@ def main():
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads)
+        cpu = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads, args.coarse_sweeps)
 
     out = {
         "metric": "DOF/s to ||r||/||f|| <= 1e-10 (plain FP64 F-cycle, 1025x1025x513)",
@@ -161,6 +171,7 @@ def main():
         "config": {"workload": "poisson-fcycle-%dx%dx%d" % dims, "grid": list(dims), "levels": args.levels,
                    "cycle": "F", "smoother": "red-black GS omega=1, %d sweeps" % args.nsmooth,
                    "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
+                   "coarsest_sweeps": args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "rbgs_sweep (fine level)", "achieved": sweep_gbs,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
@@ -171,7 +182,7 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def cpu_baseline(dims, levels, h, nsmooth, cycles, threads):
+def cpu_baseline(dims, levels, h, nsmooth, cycles, threads, coarse_sweeps):
     """The oracle (this build's C restatement, OpenMP) on the host cores: one
     F-cycle plus the residual check at the full grid, scaled to the GPU's
     cycle count to the tolerance."""
@@ -181,7 +192,7 @@ def cpu_baseline(dims, levels, h, nsmooth, cycles, threads):
         threads = len(os.sched_getaffinity(0))
     log("bench: CPU baseline (oracle, %d threads): setup" % threads)
     o = O.Solver(dims, levels, h, (0.0, 0.0, 0.0), "NNNNDD", O.CONSISTENT, O.GAUSS_SEIDEL, O.FULL_WEIGHTING, 2,
-                 threads=threads)
+                 threads=threads, coarse_sweeps=coarse_sweeps)
     o.to_device()
     t0 = time.perf_counter()
     o.solve(O.FCYCLE, nsmooth, 1.0, maxiter=1, miniter=1, tol=0.0)

@@ -43,7 +43,7 @@ enum { MGP_NORTH = 0, MGP_SOUTH = 1, MGP_EAST = 2, MGP_WEST = 3, MGP_TOP = 4, MG
 enum { MGP_DIRICHLET = 0, MGP_NEUMANN = 1 };
 enum { MGP_MODE_REFERENCE = 0, MGP_MODE_CONSISTENT = 1 };
 enum { MGP_INJECTION = 0, MGP_FULL_WEIGHTING = 1 };
-enum { MGP_F64 = 0, MGP_F32 = 1 };
+enum { MGP_F64 = 0, MGP_F32 = 1, MGP_MIXED = 2 /* FP32 cycles in FP64 refinement (solver level) */ };
 
 typedef struct mgp_array {
     void* data;        /* device pointer to the first element of the padded buffer */
@@ -93,6 +93,14 @@ int mgp_add_f32(const mgp_array* a, const mgp_array* b, void* stream);
 int mgp_norm_inf_f64(const mgp_array* a, double* norm_dev, void* stream);
 int mgp_norm_inf_f32(const mgp_array* a, double* norm_dev, void* stream);
 int mgp_zero_scalars(double* dev, int64_t n, void* stream);
+
+/* ------------------------------------------- mixed precision (no counterpart
+ * in the reference: its T is one type throughout) */
+/* out = (float)in, out = (double)in, over the shape */
+int mgp_cast_f64_f32(const mgp_array* in, const mgp_array* out, void* stream);
+int mgp_cast_f32_f64(const mgp_array* in, const mgp_array* out, void* stream);
+/* u += (double)e over the shape: the FP64 update of iterative refinement */
+int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* stream);
 
 /* ------------------------------------------------------------ smoothers */
 /* One red (colour 0) or black (colour 1) half-sweep in place over the box:
@@ -193,7 +201,8 @@ typedef struct mgp_solver_config {
     double h;            /* Settings::h */
     double origin[3];    /* Settings::origin */
     int32_t bc[6];       /* BoundaryCondition, Location_t order N,S,E,W,T,B: MGP_DIRICHLET / MGP_NEUMANN */
-    int32_t dtype;       /* T: MGP_F64 / MGP_F32 */
+    int32_t dtype;       /* T: MGP_F64 / MGP_F32, or MGP_MIXED: RefinementSolver (FP32 cycles
+                            inside FP64 iterative refinement; needs MGP_MODE_CONSISTENT) */
     int32_t restriction; /* R: MGP_INJECTION / MGP_FULL_WEIGHTING */
     int32_t relaxation;  /* S: MGP_JACOBI / MGP_GAUSS_SEIDEL */
     int32_t mode;        /* MGP_MODE_REFERENCE / MGP_MODE_CONSISTENT */
@@ -201,6 +210,7 @@ typedef struct mgp_solver_config {
     int64_t maxiter;     /* Settings::maxiter */
     int64_t miniter;     /* Settings::miniter */
     double tolerance;    /* Settings::tolerance */
+    int64_t coarse_sweeps; /* Settings::coarse_sweeps (consistent mode; 0 = nsmooth) */
 } mgp_solver_config;
 
 /* PoissonSolver(settings, BC) (libpoisson.h:79-87) */
@@ -238,6 +248,16 @@ int mgp_solver_face(void* solver, int level, int loc, void** host, mgp_array* de
 int mgp_solver_save(void* solver, const char* path);
 /* message of the last failing mgp_solver_* call on this thread */
 const char* mgp_last_error(void);
+
+/* ---------------------------------------------------------------- tuning
+ * Launch-shape knobs (process-wide; take effect for the next launches and for
+ * graphs recorded afterwards).  Keys: "sweep.target_wgs" (workgroups a
+ * one-pass sweep aims for, default 4096), "sweep.min_chunk" (shortest x march
+ * per workgroup, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup,
+ * default 8).  No counterpart in the reference, whose launch
+ * shape was fixed by the OpenMP runtime. */
+int mgp_tuning_set(const char* key, int64_t value);
+int mgp_tuning_get(const char* key, int64_t* value);
 
 /* ---------------------------------------------------------------- info */
 const char* mgp_error_string(int code);

@@ -14,6 +14,7 @@
 
 #include "mgp.h"
 #include "mgp_device.h"
+#include "mgp_sweep.h"
 
 namespace mgp {
 
@@ -47,6 +48,23 @@ __device__ __forceinline__ bool box_point(const Box& b, int64_t zchunks, int64_t
     const int64_t bx = blockIdx.x;
     i = b.x0 + bx / zchunks;
     k = b.z0 + (bx % zchunks) * BZ + threadIdx.x;
+    j = b.y0 + (int64_t)blockIdx.y * BY + threadIdx.y;
+    return k < b.z1 && j < b.y1;
+}
+
+// Column geometry for reductions: one block per (z-chunk, y-group) column,
+// marching over every x plane, so a whole reduction issues one atomic per
+// column instead of one per plane.
+static inline BoxGrid column_grid(const Box& b) {
+    BoxGrid g;
+    g.zchunks = (b.z1 - b.z0 + BZ - 1) / BZ;
+    g.grid = dim3((unsigned)g.zchunks, (unsigned)((b.y1 - b.y0 + BY - 1) / BY), 1);
+    g.block = dim3(BZ, BY, 1);
+    return g;
+}
+
+__device__ __forceinline__ bool column_point(const Box& b, int64_t& j, int64_t& k) {
+    k = b.z0 + (int64_t)blockIdx.x * BZ + threadIdx.x;
     j = b.y0 + (int64_t)blockIdx.y * BY + threadIdx.y;
     return k < b.z1 && j < b.y1;
 }
@@ -92,11 +110,27 @@ __global__ void k_add(View<T> a, View<T> b, Box bx, int64_t zc) {
 
 // include/devicearray.h:211-237: max |a| over the shape
 template <class T>
-__global__ void k_norm(View<T> a, Box bx, int64_t zc, unsigned long long* out) {
-    int64_t i, j, k;
+__global__ void k_norm(View<T> a, Box bx, unsigned long long* out) {
+    int64_t j, k;
     double v = 0.0;
-    if (box_point(bx, zc, i, j, k)) v = fabs((double)a.at(i, j, k));
+    if (column_point(bx, j, k))
+        for (int64_t i = bx.x0; i < bx.x1; i++) v = fmax(v, fabs((double)a.at(i, j, k)));
     block_max_into(v, out);
+}
+
+// mixed precision: out(i,j,k) = (To)in(i,j,k) over the shape
+template <class Ti, class To>
+__global__ void k_cast(View<Ti> in, View<To> out, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    out.at(i, j, k) = (To)in.at(i, j, k);
+}
+
+// mixed precision: u(i,j,k) += (double)e(i,j,k) over the shape
+__global__ void k_add_f32_to_f64(View<double> u, View<float> e, Box bx, int64_t zc) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    u.at(i, j, k) += (double)e.at(i, j, k);
 }
 
 __global__ void k_zero_scalars(double* p, int64_t n) {
@@ -157,38 +191,6 @@ __global__ void k_jacobi(View<T> u, View<T> v, View<T> f, Box bx, int64_t zc, T 
                                          f.at(i, j, k), one_omega, omega_sixth, hsq);
 }
 
-// Red pass of an out-of-place sweep: red points of u_out = GS update from
-// u_in, black points copied; ghosts formed on the fly from u_in.
-template <class T, int MODE>
-__global__ void k_sweep_red(View<T> ui, View<T> uo, View<T> f, Faces<T> F, Box bx, int64_t zc, T one_omega,
-                            T omega_sixth, T hsq, int64_t lin0) {
-    int64_t i, j, k;
-    if (!box_point(bx, zc, i, j, k)) return;
-    if (colour_of<MODE>(lin0, ui.s0, ui.s1, i, j, k) != 0) {
-        uo.at(i, j, k) = ui.at(i, j, k);
-        return;
-    }
-    const T vW = neighbour(ui, F, i, j, k, 0, -1), vE = neighbour(ui, F, i, j, k, 0, 1);
-    const T vS = neighbour(ui, F, i, j, k, 1, -1), vN = neighbour(ui, F, i, j, k, 1, 1);
-    const T vB = neighbour(ui, F, i, j, k, 2, -1), vT = neighbour(ui, F, i, j, k, 2, 1);
-    uo.at(i, j, k) = relax_point<T, MODE>(ui.at(i, j, k), vW, vE, vS, vN, vB, vT, f.at(i, j, k), one_omega,
-                                          omega_sixth, hsq);
-}
-
-// Black pass in place on u (its red neighbours are not written here).
-template <class T, int MODE>
-__global__ void k_sweep_black(View<T> u, View<T> f, Faces<T> F, Box bx, int64_t zc, T one_omega, T omega_sixth, T hsq,
-                              int64_t lin0) {
-    int64_t i, j, k;
-    if (!box_point(bx, zc, i, j, k)) return;
-    if (colour_of<MODE>(lin0, u.s0, u.s1, i, j, k) != 1) return;
-    const T vW = neighbour(u, F, i, j, k, 0, -1), vE = neighbour(u, F, i, j, k, 0, 1);
-    const T vS = neighbour(u, F, i, j, k, 1, -1), vN = neighbour(u, F, i, j, k, 1, 1);
-    const T vB = neighbour(u, F, i, j, k, 2, -1), vT = neighbour(u, F, i, j, k, 2, 1);
-    u.at(i, j, k) = relax_point<T, MODE>(u.at(i, j, k), vW, vE, vS, vN, vB, vT, f.at(i, j, k), one_omega, omega_sixth,
-                                         hsq);
-}
-
 // ------------------------------------------------------------- residual
 
 // include/residual.h:58-79.  Reference: r = f - (sum_nb - 6.0*u)/hsq with the
@@ -215,21 +217,24 @@ __global__ void k_residual(View<T> u, View<T> f, View<T> r, Box bx, int64_t zc, 
 }
 
 // Residual on the box plus max |r| over r's whole shape in the same pass
-// (the grid covers the shape; points outside the box only contribute).
+// (the columns cover the shape; points outside the box only contribute).
 template <class T, int MODE>
-__global__ void k_residual_norm(View<T> u, View<T> f, View<T> r, Box shape, int64_t zc, Box bx, T hsq, T inv_h2,
+__global__ void k_residual_norm(View<T> u, View<T> f, View<T> r, Box shape, Box bx, T hsq, T inv_h2,
                                 unsigned long long* out) {
-    int64_t i, j, k;
+    int64_t j, k;
     double v = 0.0;
-    if (box_point(shape, zc, i, j, k)) {
-        T rv;
-        if (i >= bx.x0 && i < bx.x1 && j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1) {
-            rv = residual_point<T, MODE>(u, u.ix(i, j, k), f.at(i, j, k), hsq, inv_h2);
-            r.at(i, j, k) = rv;
-        } else {
-            rv = r.at(i, j, k);
+    if (column_point(shape, j, k)) {
+        const bool col_in = j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1;
+        for (int64_t i = shape.x0; i < shape.x1; i++) {
+            T rv;
+            if (col_in && i >= bx.x0 && i < bx.x1) {
+                rv = residual_point<T, MODE>(u, u.ix(i, j, k), f.at(i, j, k), hsq, inv_h2);
+                r.at(i, j, k) = rv;
+            } else {
+                rv = r.at(i, j, k);
+            }
+            v = fmax(v, fabs((double)rv));
         }
-        v = fabs((double)rv);
     }
     block_max_into(v, out);
 }
@@ -463,8 +468,8 @@ template <class T>
 static int norm_inf(const mgp_array* a, double* out, void* stream) {
     if (!valid_array(a) || !out) return MGP_ERR_ARG;
     const Box bx = shape_box(a);
-    const BoxGrid g = box_grid(bx);
-    hipLaunchKernelGGL(k_norm<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*a), bx, g.zchunks,
+    const BoxGrid g = column_grid(bx);
+    hipLaunchKernelGGL(k_norm<T>, g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*a), bx,
                        (unsigned long long*)out);
     return launch_status();
 }
@@ -527,24 +532,7 @@ static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const 
         (bx.z1 == ui->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
         return MGP_ERR_ARG;
     if (box_empty(bx)) return MGP_OK;
-    const BoxGrid g = box_grid(bx);
-    const RelaxConsts<T> c(omega, faces->h);
-    const Faces<T> F = faces_of<T>(*faces);
-    const hipStream_t s = (hipStream_t)stream;
-    if (mode == MGP_MODE_REFERENCE) {
-        hipLaunchKernelGGL((k_sweep_red<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*ui),
-                           view_of<T>(*uo), view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq,
-                           lin0_of(ui));
-        hipLaunchKernelGGL((k_sweep_black<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*uo),
-                           view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, lin0_of(uo));
-    } else {
-        hipLaunchKernelGGL((k_sweep_red<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*ui),
-                           view_of<T>(*uo), view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq,
-                           lin0_of(ui));
-        hipLaunchKernelGGL((k_sweep_black<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*uo),
-                           view_of<T>(*f), F, bx, g.zchunks, c.one_omega, c.omega_sixth, c.hsq, lin0_of(uo));
-    }
-    return launch_status();
+    return march_sweep<T>(mode, ui, uo, f, faces, b, omega, march_xchunk(b), stream);
 }
 
 template <class T>
@@ -580,15 +568,13 @@ static int residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_
     const hipStream_t s = (hipStream_t)stream;
     if (norm) {
         const Box sh = shape_box(r);
-        const BoxGrid g = box_grid(sh);
+        const BoxGrid g = column_grid(sh);
         if (mode == MGP_MODE_REFERENCE)
             hipLaunchKernelGGL((k_residual_norm<T, MGP_MODE_REFERENCE>), g.grid, g.block, 0, s, view_of<T>(*u),
-                               view_of<T>(*f), view_of<T>(*r), sh, g.zchunks, bx, hsq, inv_h2,
-                               (unsigned long long*)norm);
+                               view_of<T>(*f), view_of<T>(*r), sh, bx, hsq, inv_h2, (unsigned long long*)norm);
         else
             hipLaunchKernelGGL((k_residual_norm<T, MGP_MODE_CONSISTENT>), g.grid, g.block, 0, s, view_of<T>(*u),
-                               view_of<T>(*f), view_of<T>(*r), sh, g.zchunks, bx, hsq, inv_h2,
-                               (unsigned long long*)norm);
+                               view_of<T>(*f), view_of<T>(*r), sh, bx, hsq, inv_h2, (unsigned long long*)norm);
     } else {
         if (box_empty(bx)) return MGP_OK;
         const BoxGrid g = box_grid(bx);
@@ -719,6 +705,33 @@ int mgp_add_f64(const mgp_array* a, const mgp_array* b, void* s) { return add<do
 int mgp_add_f32(const mgp_array* a, const mgp_array* b, void* s) { return add<float>(a, b, s); }
 int mgp_norm_inf_f64(const mgp_array* a, double* o, void* s) { return norm_inf<double>(a, o, s); }
 int mgp_norm_inf_f32(const mgp_array* a, double* o, void* s) { return norm_inf<float>(a, o, s); }
+
+int mgp_cast_f64_f32(const mgp_array* in, const mgp_array* out, void* s) {
+    if (!valid_array(in) || !valid_array(out) || !same_shape(in, out)) return MGP_ERR_ARG;
+    const Box bx = shape_box(in);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL((k_cast<double, float>), g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*in),
+                       view_of<float>(*out), bx, g.zchunks);
+    return launch_status();
+}
+
+int mgp_cast_f32_f64(const mgp_array* in, const mgp_array* out, void* s) {
+    if (!valid_array(in) || !valid_array(out) || !same_shape(in, out)) return MGP_ERR_ARG;
+    const Box bx = shape_box(in);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL((k_cast<float, double>), g.grid, g.block, 0, (hipStream_t)s, view_of<float>(*in),
+                       view_of<double>(*out), bx, g.zchunks);
+    return launch_status();
+}
+
+int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* s) {
+    if (!valid_array(u) || !valid_array(e) || !same_shape(u, e)) return MGP_ERR_ARG;
+    const Box bx = shape_box(u);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL(k_add_f32_to_f64, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*u), view_of<float>(*e),
+                       bx, g.zchunks);
+    return launch_status();
+}
 
 int mgp_zero_scalars(double* p, int64_t n, void* s) {
     if (!p || n < 0) return MGP_ERR_ARG;

@@ -5,7 +5,10 @@
 // capture of whole cycles, and events for timing.
 #include <hip/hip_runtime.h>
 
+#include <cstring>
+
 #include "mgp.h"
+#include "mgp_sweep.h"
 
 static inline int st(hipError_t e) { return e == hipSuccess ? MGP_OK : (int)e; }
 
@@ -111,6 +114,24 @@ int mgp_event_sync(void* event) { return st(hipEventSynchronize((hipEvent_t)even
 int mgp_event_elapsed_ms(void* start, void* stop, float* ms) {
     if (!ms) return MGP_ERR_ARG;
     return st(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+}
+
+int mgp_tuning_set(const char* key, int64_t value) {
+    if (!key || value < 1) return MGP_ERR_ARG;
+    if (!strcmp(key, "sweep.target_wgs")) mgp::g_sweep_target_wgs = value;
+    else if (!strcmp(key, "sweep.min_chunk")) mgp::g_sweep_min_chunk = value;
+    else if (!strcmp(key, "sweep.waves") && (value == 4 || value == 8 || value == 16)) mgp::g_sweep_waves = value;
+    else return MGP_ERR_ARG;
+    return MGP_OK;
+}
+
+int mgp_tuning_get(const char* key, int64_t* value) {
+    if (!key || !value) return MGP_ERR_ARG;
+    if (!strcmp(key, "sweep.target_wgs")) *value = mgp::g_sweep_target_wgs;
+    else if (!strcmp(key, "sweep.min_chunk")) *value = mgp::g_sweep_min_chunk;
+    else if (!strcmp(key, "sweep.waves")) *value = mgp::g_sweep_waves;
+    else return MGP_ERR_ARG;
+    return MGP_OK;
 }
 
 const char* mgp_error_string(int code) {

@@ -80,6 +80,59 @@ struct SolverImpl : SolverBase {
     void save(const char* path) override { s.save(0, path); }
 };
 
+// FP32 cycles inside FP64 refinement; host arrays are the FP64 ones
+template <template <class> class R, template <class> class S>
+struct MixedImpl : SolverBase {
+    Poisson::RefinementSolver<R, Poisson::TrilinearInterpolation, S> s;
+    MixedImpl(Poisson::Settings& st, Poisson::BoundaryCondition& bc) : s(st, bc) {}
+    void init_fn(Poisson::funptr u, Poisson::funptr f, Poisson::funptr dx, Poisson::funptr dy,
+                 Poisson::funptr dz) override {
+        s.init(u, f, dx, dy, dz);
+    }
+    void init_zero() override { s.init_zero(); }
+    void to_device() override { s.to_device(); }
+    void to_host() override { s.to_host(); }
+    void solve(const std::string& c, int64_t nsmooth, double omega, double maxtime) override {
+        s.solve(c, nsmooth, omega, "", maxtime);
+    }
+    void cycle(const std::string& c, int64_t nsmooth, double omega) override {
+        (void)c, (void)nsmooth, (void)omega;
+        throw std::invalid_argument("a refinement solver has no bare cycle; use solve");
+    }
+    double relres() override { return s.fp64().relres(); }
+    void sync() override { Poisson::mgpx::check(mgp_stream_sync(s.get_stream()), "sync"); }
+    void reset() override { s.reset_solution(); }
+    void stats(double* rr, double* t, int64_t* it) override {
+        if (rr) *rr = s.relative_residual();
+        if (t) *t = s.solve_time();
+        if (it) *it = (int64_t)s.solve_iterations();
+    }
+    int array(int level, int which, void** host, mgp_array* dev) override {
+        if (level != 0) return MGP_ERR_ARG;  // the FP64 problem lives on the finest level only
+        Poisson::Domain<double>& d = s.fp64().get_domain(0, 0);
+        Poisson::DeviceArray<double>* a = nullptr;
+        switch (which) {
+        case 'u': a = d.u; break;
+        case 'p': a = d.uprev; break;
+        case 'f': a = d.f; break;
+        case 'r': a = d.r; break;
+        default: return MGP_ERR_ARG;
+        }
+        if (host) *host = a->at;
+        if (dev) *dev = a->desc();
+        return MGP_OK;
+    }
+    int face(int level, int loc, void** host, mgp_array* dev, int* type) override {
+        if (level != 0 || loc < 0 || loc > 5) return MGP_ERR_ARG;
+        Poisson::Boundary<double>* b = s.fp64().get_domain(0, 0).face(loc);
+        if (host) *host = b->arr.at;
+        if (dev) *dev = b->arr.desc();
+        if (type) *type = b->is_non_eliminated() ? MGP_DIRICHLET : MGP_NEUMANN;
+        return MGP_OK;
+    }
+    void save(const char* path) override { s.fp64().save(0, path); }
+};
+
 template <class T>
 SolverBase* make_t(const mgp_solver_config* c, Poisson::Settings& st, Poisson::BoundaryCondition& bc) {
     const bool fw = c->restriction == MGP_FULL_WEIGHTING;
@@ -117,7 +170,7 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
     *out = nullptr;
     for (int q = 0; q < 6; q++)
         if (c->bc[q] != MGP_DIRICHLET && c->bc[q] != MGP_NEUMANN) return MGP_ERR_ARG;
-    if ((c->dtype != MGP_F64 && c->dtype != MGP_F32) ||
+    if ((c->dtype != MGP_F64 && c->dtype != MGP_F32 && c->dtype != MGP_MIXED) ||
         (c->restriction != MGP_INJECTION && c->restriction != MGP_FULL_WEIGHTING) ||
         (c->relaxation != MGP_JACOBI && c->relaxation != MGP_GAUSS_SEIDEL) ||
         (c->mode != MGP_MODE_REFERENCE && c->mode != MGP_MODE_CONSISTENT))
@@ -135,13 +188,22 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
         st.maxiter = c->maxiter;
         st.miniter = c->miniter;
         st.tolerance = c->tolerance;
+        st.coarse_sweeps = c->coarse_sweeps;
         st.mode = c->mode;
         st.use_graphs = !(c->flags & MGP_SOLVER_NO_GRAPHS);
         st.fused_sweeps = !(c->flags & MGP_SOLVER_NO_FUSED);
         st.print_cycle_time = !(c->flags & MGP_SOLVER_QUIET);
         auto t = [&](int q) { return c->bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
         Poisson::BoundaryCondition bc{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
-        *out = c->dtype == MGP_F64 ? make_t<double>(c, st, bc) : make_t<float>(c, st, bc);
+        if (c->dtype == MGP_MIXED) {
+            const bool fw = c->restriction == MGP_FULL_WEIGHTING, gs = c->relaxation == MGP_GAUSS_SEIDEL;
+            if (fw) *out = gs ? (SolverBase*)new MixedImpl<Poisson::FullWeighting, Poisson::GaussSeidel>(st, bc)
+                              : (SolverBase*)new MixedImpl<Poisson::FullWeighting, Poisson::Jacobi>(st, bc);
+            else *out = gs ? (SolverBase*)new MixedImpl<Poisson::Injection, Poisson::GaussSeidel>(st, bc)
+                           : (SolverBase*)new MixedImpl<Poisson::Injection, Poisson::Jacobi>(st, bc);
+        } else {
+            *out = c->dtype == MGP_F64 ? make_t<double>(c, st, bc) : make_t<float>(c, st, bc);
+        }
     });
 }
 

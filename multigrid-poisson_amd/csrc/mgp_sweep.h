@@ -1,0 +1,19 @@
+// One-pass red-black sweep (mgp_sweep.hip), called by mgp_rbgs_sweep_*.
+#pragma once
+
+#include <stdint.h>
+
+#include "mgp.h"
+
+namespace mgp {
+// Arguments validated by the caller (checkerboard colouring, box inside the
+// shape, Neumann faces wherever the box touches the boundary).
+template <class T>
+int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                const int64_t box[6], double omega, int64_t xchunk, void* stream);
+// default x-chunk length for a box
+int64_t march_xchunk(const int64_t box[6]);
+extern int64_t g_sweep_target_wgs;
+extern int64_t g_sweep_min_chunk;
+extern int64_t g_sweep_waves;
+}  // namespace mgp

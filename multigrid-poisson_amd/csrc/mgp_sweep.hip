@@ -1,0 +1,403 @@
+// One-pass red-black Gauss-Seidel sweep for gfx950 (mgp_rbgs_sweep_*).
+//
+// The reference runs a sweep as two in-place half-sweeps over the whole
+// level, each preceded by a Neumann ghost refresh (include/relaxation.h:25-90,
+// include/gaussseidel.h:28-94): every point is read and written twice.  Here a
+// workgroup owns a (y, z) tile and marches along x; at plane i it updates the
+// red points of plane i (on the tile plus a one-point ring, from the old
+// values) and then the black points of plane i-1 on the tile (from the new
+// red values), writing both colours of plane i-1 to the output buffer.  The
+// ring's red values are recomputed by the neighbouring tiles as well, so tiles
+// never wait for each other; the output goes to a second buffer so no tile
+// can see another tile's new values early.  Per point the arithmetic is the
+// reference's (or the consistent mode's) update, and ghost values are formed
+// exactly as the ghost refresh would have stored them, so the result equals
+// the two half-sweeps bit for bit.
+//
+// Tile: 64 z-columns by 16 y-rows of ring positions per plane (62 x 14
+// interior); 512 threads (8 waves), each owning 2 ring positions (2 rows of one
+// column) in registers.  LDS holds the old plane (with a 2-point ring, for the
+// red stencil) and two planes of new red values (ping-pong).  The loop body is
+// branch-free per lane: positions outside the box keep their value through a
+// select, Neumann ghosts are folded into the loaded values (old plane) or
+// written into LDS by the ghost positions' owners (new red plane), and every
+// load is unconditional from a valid address, so the compiler can count
+// outstanding memory operations exactly and the next plane's loads overlap a
+// whole step.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdint>
+
+#include "mgp.h"
+#include "mgp_device.h"
+#include "mgp_sweep.h"
+
+namespace mgp {
+
+namespace {
+
+constexpr int RW = 64;        // ring-1 rectangle width (z)
+constexpr int RH = 16;        // ring-1 rectangle height (y)
+constexpr int TZ = RW - 2;    // interior columns per tile
+constexpr int TY = RH - 2;    // interior rows per tile
+constexpr int UW = RW + 2;    // old-plane rectangle (2-point ring) width
+constexpr int UH = RH + 2;    // ... height
+constexpr int NBORDER = 2 * UW + 2 * RH;  // outermost ring of the old-plane rectangle
+
+template <class T>
+struct MarchArgs {
+    View<T> ui, uo, f;
+    const T* g[6];
+    T two_h;
+    int64_t bx0, bx1, by0, by1, bz0, bz1;
+    int64_t lin0;
+    T one_omega, omega_sixth, hsq;
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t xchunk;
+};
+
+// element at a 32-bit byte offset from a wave-uniform base: lets the compiler
+// use the scalar-base + 32-bit vector-offset addressing form
+template <class T>
+__device__ __forceinline__ T ld(const T* base, uint32_t byte_off) {
+    return *reinterpret_cast<const T*>(reinterpret_cast<const char*>(base) + byte_off);
+}
+template <class T>
+__device__ __forceinline__ void st(T* base, uint32_t byte_off, T v) {
+    *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + byte_off) = v;
+}
+
+template <class T, int MODE>
+__device__ __forceinline__ T update(T vc, T vW, T vE, T vS, T vN, T vB, T vT, T fc, T one_omega, T omega_sixth, T hsq) {
+    if (MODE == MGP_MODE_REFERENCE) {
+        return one_omega * vc + omega_sixth * (vW + vE + vS + vN + vB + vT - hsq * fc);
+    } else {
+        const T s = ((vW - vc) + (vE - vc)) + ((vS - vc) + (vN - vc)) + ((vB - vc) + (vT - vc));
+        return fma(omega_sixth, fma(-hsq, fc, s), vc);
+    }
+}
+
+// In-plane Neumann ghost of a ring position: sign (+-1; 0 = not a ghost),
+// mirror row/column offset in the tile, face array and its element for plane 0
+// and step per plane (face layouts (nx,1,nz) and (nx,ny,1)).
+template <class T>
+struct Ghost {
+    T sign;
+    int dr, dc;  // mirror = (rr + dr, cc + dc)
+    const T* g;
+    int64_t gstride;
+};
+
+template <class T>
+__device__ __forceinline__ Ghost<T> ghost_of(const MarchArgs<T>& A, int64_t n1, int64_t n2, int64_t y, int64_t z) {
+    Ghost<T> gh;
+    gh.sign = 0;
+    gh.dr = gh.dc = 0;
+    gh.g = nullptr;
+    gh.gstride = 0;
+    if (z >= 0 && z < n2 && (y == -1 || y == n1) && A.g[y < 0 ? MGP_SOUTH : MGP_NORTH]) {
+        gh.sign = y < 0 ? (T)-1.0 : (T)1.0;
+        gh.dr = y < 0 ? 2 : -2;
+        gh.g = A.g[y < 0 ? MGP_SOUTH : MGP_NORTH] + z;
+        gh.gstride = n2;
+    } else if (y >= 0 && y < n1 && (z == -1 || z == n2) && A.g[z < 0 ? MGP_BOTTOM : MGP_TOP]) {
+        gh.sign = z < 0 ? (T)-1.0 : (T)1.0;
+        gh.dc = z < 0 ? 2 : -2;
+        gh.g = A.g[z < 0 ? MGP_BOTTOM : MGP_TOP] + y;
+        gh.gstride = n1;
+    }
+    return gh;
+}
+
+// NQ ring rows per thread; RH / NQ waves per workgroup
+template <class T, int MODE, int NQ>
+__global__ void __launch_bounds__(64 * (RH / NQ)) k_rbgs_march(MarchArgs<T> A) {
+    constexpr int WAVES = RH / NQ;
+    static_assert(64 * WAVES >= NBORDER, "too few threads for the border");
+    __shared__ T sU[UH][UW];
+    __shared__ T sP[2][RH][RW];
+
+    // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
+    // give each XCD a contiguous run of tiles (neighbours share halo rows in
+    // that XCD's L2)
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * TY;
+    const int64_t z0 = A.bz0 + (int64_t)tz * TZ;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+
+    const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2, s0 = A.ui.s0, s1 = A.ui.s1;
+    const T two_h = A.two_h;
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int cc = lane + 1;
+    const int64_t z = z0 - 1 + lane;
+    const bool z_in = z >= A.bz0 && z < A.bz1;
+    const bool z_shape = z >= 0 && z < n2;
+    const bool z_core = lane >= 1 && lane <= TZ;
+
+    // per owned position: element offset within a plane (clamped into the
+    // shape so every load is valid), and masks
+    uint32_t off[NQ], foff[NQ];  // element offset in a plane; index in an x-face array
+    bool inb[NQ], core[NQ], inshape[NQ];
+    int col0[NQ];
+#pragma unroll
+    for (int q = 0; q < NQ; q++) {
+        const int rr = 1 + wave * NQ + q;
+        const int64_t y = y0 - 2 + rr;
+        inshape[q] = z_shape && y >= 0 && y < n1;
+        inb[q] = z_in && y >= A.by0 && y < A.by1;
+        core[q] = inb[q] && z_core && rr >= 2 && rr <= TY + 1;
+        const int64_t yc = min(max(y, (int64_t)0), n1 - 1), zc = min(max(z, (int64_t)0), n2 - 1);
+        off[q] = (uint32_t)((yc * s1 + zc) * (int64_t)sizeof(T));   // bytes
+        foff[q] = (uint32_t)((yc * n2 + zc) * (int64_t)sizeof(T));  // bytes
+        // colour at plane 0 (reference: parity of the halo-inclusive index,
+        // include/gaussseidel.h:56,81; consistent: parity of i+j+k)
+        col0[q] = MODE == MGP_MODE_REFERENCE ? (int)((A.lin0 + y * s1 + z) & 1) : (int)((y + z) & 1);
+    }
+    // border position of this thread (outermost ring of the old-plane rectangle)
+    int brr = 0, bcc = 0;
+    bool has_border = false;
+    {
+        const int b = threadIdx.x;
+        if (b < UW) brr = 0, bcc = b, has_border = true;
+        else if (b < 2 * UW) brr = UH - 1, bcc = b - UW, has_border = true;
+        else if (b < 2 * UW + RH) brr = 1 + (b - 2 * UW), bcc = 0, has_border = true;
+        else if (b < NBORDER) brr = 1 + (b - 2 * UW - RH), bcc = UW - 1, has_border = true;
+    }
+    const int64_t by = y0 - 2 + brr, bz = z0 - 2 + bcc;
+    const bool b_shape = has_border && by >= 0 && by < n1 && bz >= 0 && bz < n2;
+    const int64_t byc = min(max(by, (int64_t)0), n1 - 1), bzc = min(max(bz, (int64_t)0), n2 - 1);
+    const uint32_t boff = (uint32_t)((byc * s1 + bzc) * (int64_t)sizeof(T));
+    const uint32_t bfoff = (uint32_t)((byc * n2 + bzc) * (int64_t)sizeof(T));
+    // Neumann ghost positions of this tile (uniform: only boundary tiles)
+    const bool yz_ghosts = (y0 == 0 && A.g[MGP_SOUTH]) || (y0 + TY >= n1 && A.g[MGP_NORTH]) ||
+                           (z0 == 0 && A.g[MGP_BOTTOM]) || (z0 + TZ >= n2 && A.g[MGP_TOP]);
+    const T* gW = A.g[MGP_WEST];
+    const T* gE = A.g[MGP_EAST];
+
+    // plane j of the old values at the owned positions and at the border
+    // position; the x-ghost planes -1 and n0 (Neumann west/east) are
+    // sign*2h*g + the mirror plane (include/neumann.h:123-124 operand order),
+    // formed at the two steps that need them
+    const T* up = A.ui.p;
+    auto plane_of = [&](int64_t j) {
+        return up + min(max(j < 0 ? (int64_t)1 : (j >= n0 ? n0 - 2 : j), (int64_t)0), n0 - 1) * s0;
+    };
+    auto xghost = [&](int64_t j, uint32_t fo, T v) -> T {  // j outside [0, n0): uniform
+        const T* g = j < 0 ? gW : gE;
+        if (!g) return v;
+        const T sg = j < 0 ? (T)-1.0 : (T)1.0;
+        return sg * two_h * ld(g, fo) + v;  // face (1, ny, nz): y*nz + z
+    };
+    auto load_owned = [&](int64_t j, T (&dst)[NQ]) {
+        const T* pl = plane_of(j);
+#pragma unroll
+        for (int q = 0; q < NQ; q++) dst[q] = inshape[q] ? ld(pl, off[q]) : (T)0.0;
+        if (j < 0 || j >= n0) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) dst[q] = inshape[q] ? xghost(j, foff[q], dst[q]) : (T)0.0;
+        }
+    };
+    auto load_border = [&](int64_t j) -> T {
+        const T v = b_shape ? ld(plane_of(j), boff) : (T)0.0;
+        return (j < 0 || j >= n0) && b_shape ? xghost(j, bfoff, v) : v;
+    };
+    const T* fpl = A.f.p;
+    auto load_f = [&](int64_t j, T (&dst)[NQ]) {
+        const T* pl = fpl + min(max(j, (int64_t)0), n0 - 1) * s0;
+#pragma unroll
+        for (int q = 0; q < NQ; q++) dst[q] = ld(pl, off[q]);
+    };
+
+    T uo_m[NQ], uo_c[NQ], uo_p[NQ], uo_n[NQ], f_m[NQ], f_c[NQ], f_n[NQ], Pm2[NQ], Pm1[NQ], Pc[NQ];
+    const int64_t i0 = xa - 1;
+    load_owned(i0 - 1, uo_m);
+    load_owned(i0, uo_c);
+    load_owned(i0 + 1, uo_p);
+    load_owned(i0 + 2, uo_n);
+    T bd_c = load_border(i0), bd_n = load_border(i0 + 1);
+    load_f(i0, f_c);
+    load_f(i0 + 1, f_n);
+#pragma unroll
+    for (int q = 0; q < NQ; q++) f_m[q] = Pm2[q] = Pm1[q] = Pc[q] = 0;
+
+    for (int64_t i = i0; i <= xb; i++) {
+        const bool plane_in = i >= 0 && i < n0;  // uniform
+        const bool x_in = i >= A.bx0 && i < A.bx1;
+        const int par = (int)(i & 1);
+        // A: old plane i into LDS
+        if (plane_in) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) sU[1 + wave * NQ + q][cc] = uo_c[q];
+            if (has_border) sU[brr][bcc] = bd_c;
+        }
+        __syncthreads();
+        if (yz_ghosts && plane_in) {
+            // in-plane ghosts of the old plane (owned and border positions):
+            // sign*2h*g + mirror, the values the ghost refresh before the red
+            // half-sweep stores
+            T gv[NQ];
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = 1 + wave * NQ + q;
+                const Ghost<T> gh = ghost_of(A, n1, n2, y0 - 2 + rr, z);
+                gv[q] = gh.sign != (T)0.0 ? gh.sign * two_h * gh.g[i * gh.gstride] + sU[rr + gh.dr][cc + gh.dc]
+                                          : uo_c[q];
+            }
+            const Ghost<T> bg = ghost_of(A, n1, n2, by, bz);
+            const bool b_ghost = has_border && bg.sign != (T)0.0;
+            const T bgv = b_ghost ? bg.sign * two_h * bg.g[i * bg.gstride] + sU[brr + bg.dr][bcc + bg.dc] : (T)0.0;
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = 1 + wave * NQ + q;
+                sU[rr][cc] = gv[q];
+                uo_c[q] = gv[q];
+            }
+            if (b_ghost) sU[brr][bcc] = bgv;
+            __syncthreads();
+        }
+        // B: red points of plane i on the ring rectangle, from old values
+        if (plane_in) {
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = 1 + wave * NQ + q;
+                const T r = update<T, MODE>(uo_c[q], uo_m[q], uo_p[q], sU[rr - 1][cc], sU[rr + 1][cc],
+                                            sU[rr][cc - 1], sU[rr][cc + 1], f_c[q], A.one_omega, A.omega_sixth,
+                                            A.hsq);
+                const bool red = x_in && inb[q] && ((col0[q] ^ par) == 0);
+                Pc[q] = red ? r : uo_c[q];
+                sP[par][rr - 1][lane] = Pc[q];
+            }
+        }
+        __syncthreads();
+        if (yz_ghosts && plane_in) {
+            // in-plane ghosts of the new red plane (the refresh before the
+            // black half-sweep)
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int rr = 1 + wave * NQ + q;
+                const Ghost<T> gh = ghost_of(A, n1, n2, y0 - 2 + rr, z);
+                if (gh.sign != (T)0.0) Pc[q] = gh.sign * two_h * gh.g[i * gh.gstride] + sP[par][rr - 1 + gh.dr][lane + gh.dc];
+            }
+            __syncthreads();
+#pragma unroll
+            for (int q = 0; q < NQ; q++) sP[par][wave * NQ + q][lane] = Pc[q];
+            __syncthreads();
+        }
+        // C: black points of plane p = i-1 on the tile interior, from the new
+        // red values; both colours of plane p written out
+        const int64_t p = i - 1;
+        if (p >= xa && p < xb) {
+            const int pb = par ^ 1;
+            T* outp = A.uo.p + p * s0;
+            // x-neighbours: new red planes p-1 and p+1, or their ghosts beyond
+            // the x faces (uniform branches)
+            const bool w_ghost = p == 0 && gW, e_ghost = p == n0 - 1 && gE;
+#pragma unroll
+            for (int q = 0; q < NQ; q++) {
+                const int r0 = wave * NQ + q;
+                T vW = Pm2[q], vE = Pc[q];
+                if (w_ghost) vW = (T)-1.0 * two_h * ld(gW, foff[q]) + Pc[q];
+                if (e_ghost) vE = (T)1.0 * two_h * ld(gE, foff[q]) + Pm2[q];
+                const int rS = max(r0 - 1, 0), rN = min(r0 + 1, RH - 1);
+                const int lB = max(lane - 1, 0), lT = min(lane + 1, RW - 1);
+                const T b = update<T, MODE>(uo_m[q], vW, vE, sP[pb][rS][lane], sP[pb][rN][lane], sP[pb][r0][lB],
+                                            sP[pb][r0][lT], f_m[q], A.one_omega, A.omega_sixth, A.hsq);
+                const bool black = (col0[q] ^ pb) == 1;
+                if (core[q]) st(outp, off[q], black ? b : Pm1[q]);
+            }
+        }
+        // rotate the x window and prefetch the plane after next
+#pragma unroll
+        for (int q = 0; q < NQ; q++) {
+            uo_m[q] = uo_c[q];
+            uo_c[q] = uo_p[q];
+            uo_p[q] = uo_n[q];
+            f_m[q] = f_c[q];
+            f_c[q] = f_n[q];
+            Pm2[q] = Pm1[q];
+            Pm1[q] = Pc[q];
+        }
+        bd_c = bd_n;
+        load_owned(i + 3, uo_n);
+        bd_n = load_border(i + 2);
+        load_f(i + 2, f_n);
+    }
+}
+
+}  // namespace
+
+template <class T>
+int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                const int64_t b[6], double omega, int64_t xchunk, void* stream) {
+    MarchArgs<T> A;
+    A.ui = view_of<T>(*ui);
+    A.uo = view_of<T>(*uo);
+    A.f = view_of<T>(*f);
+    for (int q = 0; q < 6; q++) A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const T*>(faces->g[q]) : nullptr;
+    A.two_h = (T)(2.0 * faces->h);  // include/neumann.h:108
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.lin0 = ui->halo[MGP_WEST] * ui->stride[0] + ui->halo[MGP_SOUTH] * ui->stride[1] + ui->halo[MGP_BOTTOM];
+    const T om = (T)omega;  // include/gaussseidel.h:50-52
+    A.one_omega = (T)(1.0 - om);
+    A.omega_sixth = (T)(om / 6.0);
+    A.hsq = (T)(faces->h * faces->h);
+    A.ytiles = (int)((b[3] - b[2] + TY - 1) / TY);
+    A.ztiles = (int)((b[5] - b[4] + TZ - 1) / TZ);
+    const int64_t nx = b[1] - b[0];
+    if (xchunk <= 0) xchunk = nx;
+    const int xchunks = (int)((nx + xchunk - 1) / xchunk);
+    A.xchunk = xchunk;
+    A.ntiles = A.ytiles * A.ztiles * xchunks;
+    A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    const hipStream_t st = (hipStream_t)stream;
+    const dim3 g(A.ntiles_pad);
+    const bool ref = mode == MGP_MODE_REFERENCE;
+    switch (g_sweep_waves) {
+    case 4:
+        if (ref) hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_REFERENCE, 4>), g, dim3(256), 0, st, A);
+        else hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_CONSISTENT, 4>), g, dim3(256), 0, st, A);
+        break;
+    case 16:
+        if (ref) hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_REFERENCE, 1>), g, dim3(1024), 0, st, A);
+        else hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_CONSISTENT, 1>), g, dim3(1024), 0, st, A);
+        break;
+    default:
+        if (ref) hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_REFERENCE, 2>), g, dim3(512), 0, st, A);
+        else hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_CONSISTENT, 2>), g, dim3(512), 0, st, A);
+        break;
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+
+// Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
+int64_t g_sweep_target_wgs = 4096;  // workgroups to aim for per launch
+int64_t g_sweep_min_chunk = 48;     // shortest x march per workgroup
+int64_t g_sweep_waves = 8;          // waves per workgroup: 4, 8 or 16
+
+// x-chunk length: enough workgroups to keep every CU busy with several
+// marches, but chunks long enough that the two warm-up planes per chunk stay
+// cheap
+int64_t march_xchunk(const int64_t b[6]) {
+    const int64_t tiles = ((b[3] - b[2] + TY - 1) / TY) * ((b[5] - b[4] + TZ - 1) / TZ);
+    const int64_t nx = b[1] - b[0];
+    int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+    return (nx + chunks - 1) / chunks;
+}
+
+template int march_sweep<double>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
+                                 const int64_t[6], double, int64_t, void*);
+template int march_sweep<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
+                                const int64_t[6], double, int64_t, void*);
+
+}  // namespace mgp

@@ -18,7 +18,7 @@ DIRICHLET, NEUMANN = 0, 1
 REFERENCE, CONSISTENT = 0, 1
 INJECTION, FULL_WEIGHTING = 0, 1
 JACOBI, GAUSS_SEIDEL = 0, 1
-F64, F32 = 0, 1
+F64, F32, MIXED = 0, 1, 2
 NO_GRAPHS, NO_FUSED, QUIET = 1, 2, 4
 LOC = "NSEWTB"
 
@@ -41,7 +41,7 @@ class SolverConfig(ctypes.Structure):
                 ("origin", ctypes.c_double * 3), ("bc", ctypes.c_int32 * 6), ("dtype", ctypes.c_int32),
                 ("restriction", ctypes.c_int32), ("relaxation", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flags", ctypes.c_int32), ("maxiter", ctypes.c_int64), ("miniter", ctypes.c_int64),
-                ("tolerance", ctypes.c_double)]
+                ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64)]
 
 
 _lib = None
@@ -56,9 +56,12 @@ _SIGS = {
     "mgp_memcpy_d2d": [vp, vp, ctypes.c_size_t, vp], "mgp_stream_create": [P(vp)], "mgp_stream_destroy": [vp],
     "mgp_stream_sync": [vp], "mgp_mem_info": [P(ctypes.c_size_t), P(ctypes.c_size_t)],
     "mgp_zero_scalars": [vp, i64, vp],
+    "mgp_cast_f64_f32": [P(Array), P(Array), vp], "mgp_cast_f32_f64": [P(Array), P(Array), vp],
+    "mgp_add_f32_to_f64": [P(Array), P(Array), vp],
     "mgp_graph_begin": [vp], "mgp_graph_end": [vp, P(vp)], "mgp_graph_launch": [vp, vp], "mgp_graph_destroy": [vp],
     "mgp_event_create": [P(vp)], "mgp_event_destroy": [vp], "mgp_event_record": [vp, vp], "mgp_event_sync": [vp],
     "mgp_event_elapsed_ms": [vp, vp, P(f)],
+    "mgp_tuning_set": [ctypes.c_char_p, i64], "mgp_tuning_get": [ctypes.c_char_p, P(i64)],
     "mgp_solver_create": [P(SolverConfig), P(vp)], "mgp_solver_destroy": [vp],
     "mgp_solver_init_problem": [vp, i32], "mgp_solver_init_zero": [vp], "mgp_solver_to_device": [vp],
     "mgp_solver_to_host": [vp], "mgp_solver_solve": [vp, ctypes.c_char_p, i64, d, d],
@@ -221,7 +224,7 @@ class Solver:
 
     def __init__(self, dims, levels, h, origin=(0.0, 0.0, 0.0), bc="DDDDDD", mode=REFERENCE,
                  relaxation=GAUSS_SEIDEL, restriction=INJECTION, problem=0, dtype=F64, flags=QUIET,
-                 maxiter=100, miniter=1, tolerance=1e-6):
+                 maxiter=100, miniter=1, tolerance=1e-6, coarse_sweeps=0):
         require_gpu()
         c = SolverConfig()
         c.dims[:] = [int(x) for x in dims]
@@ -232,9 +235,10 @@ class Solver:
         c.dtype, c.restriction, c.relaxation = int(dtype), int(restriction), int(relaxation)
         c.mode, c.flags = int(mode), int(flags)
         c.maxiter, c.miniter, c.tolerance = int(maxiter), int(miniter), float(tolerance)
+        c.coarse_sweeps = int(coarse_sweeps)
         self.config = c
         self.levels = int(levels)
-        self.np_dtype = np.float64 if dtype == F64 else np.float32
+        self.np_dtype = np.float32 if dtype == F32 else np.float64
         h_ = vp()
         check(lib().mgp_solver_create(ctypes.byref(c), ctypes.byref(h_)), "mgp_solver_create")
         self.h = h_.value

@@ -53,6 +53,7 @@ private:
 
     // ---- graph replay of whole cycles
     void* stream = nullptr;
+    bool owns_stream = true;
     detail::Scalar* scalar = nullptr;
     struct LevelState {
         DeviceArray<T>* u;
@@ -125,6 +126,14 @@ public:
     void cycle(const std::string solver, const int_t nsmooth = 10, T omega = -1.0);
     // residual on level 0 and ||r||inf / ||f||inf (waits)
     double relres();
+    // enqueue on `s` (owned by the caller) instead of this solver's own stream;
+    // recorded cycle graphs are dropped
+    void use_stream(void* s) {
+        mgp_stream_sync(stream);
+        clear_graphs();
+        stream = s;
+        owns_stream = false;
+    }
     // every level's u back to zero with its boundary data, on the device (waits)
     void reset_solution() {
         mgpx::StreamScope scope(stream);
@@ -169,7 +178,10 @@ PoissonSolver<T, R, P, S>::~PoissonSolver() {
         delete[] domains;
     }
     delete scalar;
-    if (stream) mgp_stream_sync(stream), mgp_stream_destroy(stream);
+    if (stream) {
+        mgp_stream_sync(stream);
+        if (owns_stream) mgp_stream_destroy(stream);
+    }
 }
 
 // libpoisson.h:113-156 on one device
@@ -472,6 +484,8 @@ Boundary<T>& PoissonSolver<T, R, P, S>::get_bottom_bc(uint_t gpuid) {
 
 // the reference's headers leave these names in the global namespace
 // (libpoisson.h:19-25, array.h:78-82); sources written against it use them
+#include "refinement.h"
+
 using std::cerr;
 using std::cout;
 using std::endl;

@@ -1,0 +1,154 @@
+// FP64 iterative refinement around FP32 multigrid cycles (addition; the
+// reference's PoissonSolver computes in one type T throughout).
+//
+// RefinementSolver<R, P, S> keeps the problem (f and the boundary data) and the
+// solution u in FP64 on the finest level and repeats
+//     r  = f - L_h u                    (FP64 residual with the real boundary data)
+//     solve L_h e = r approximately     (one FP32 V- or F-cycle from e = 0,
+//                                         homogeneous boundary data, all levels)
+//     u += e                            (FP64 update)
+// until ||r||inf / ||f||inf <= tolerance.  FP32 carries the smoothing,
+// restriction and prolongation; FP64 the residual, the update and the
+// convergence test, so the attainable accuracy is that of the FP64 residual.
+// It requires SolverMode CONSISTENT (homogeneous coarse Neumann data).
+#ifndef POISSON_REFINEMENT
+#define POISSON_REFINEMENT
+
+#include <chrono>
+#include <cmath>
+#include <iomanip>
+#include <iostream>
+#include <stdexcept>
+#include <string>
+
+namespace Poisson {
+
+template <class T, template <class> class R, template <class> class P, template <class> class S>
+class PoissonSolver;
+
+template <template <class> class R, template <class> class P, template <class> class S>
+class RefinementSolver {
+public:
+    RefinementSolver(Settings& s, BoundaryCondition& bc)
+        : settings(s), outer_settings(s), inner_settings(s), BC(bc) {
+        if (s.mode != CONSISTENT)
+            throw std::invalid_argument("RefinementSolver needs Settings::mode = CONSISTENT");
+        outer_settings.levels = 1;
+        inner_settings.print_cycle_time = false;
+        outer_settings.print_cycle_time = false;
+        outer = new PoissonSolver<double, R, P, S>(outer_settings, BC);
+        try {
+            inner = new PoissonSolver<float, R, P, S>(inner_settings, BC);
+        } catch (...) {
+            delete outer;
+            throw;
+        }
+        inner->use_stream(outer->get_stream());
+        void* p = nullptr;
+        mgpx::check(mgp_malloc(&p, 8 * sizeof(double)), "RefinementSolver");
+        dev_norm = static_cast<double*>(p);
+        mgpx::check(mgp_host_alloc(&p, 8 * sizeof(double)), "RefinementSolver");
+        host_norm = static_cast<double*>(p);
+    }
+    RefinementSolver(const RefinementSolver&) = delete;
+    RefinementSolver& operator=(const RefinementSolver&) = delete;
+    ~RefinementSolver() {
+        delete inner;
+        delete outer;
+        mgp_free(dev_norm);
+        mgp_host_free(host_norm);
+    }
+
+    void init() { this->init(&ufun, &ffun, &dudxfun, &dudyfun, &dudzfun); }
+    void init(funptr u, funptr f, funptr dx, funptr dy, funptr dz) {
+        outer->init(u, f, dx, dy, dz);
+        inner->init_zero();
+    }
+    void init_zero() {
+        outer->init_zero();
+        inner->init_zero();
+    }
+    void to_device() {
+        outer->to_device();
+        inner->init_zero();
+        inner->to_device();
+    }
+    void to_host() { outer->to_host(); }
+    void verbose(bool onoff) { is_verbose = onoff; }
+    void reset_solution() { outer->reset_solution(); }
+
+    void solve(const std::string solver = "Fcycle", const int_t nsmooth = 2, double omega = -1.0,
+               std::string ofile = "", double maxtime = 20 * 60) {
+        using clock = std::chrono::steady_clock;
+        const auto t0 = clock::now();
+        auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
+        const bool v = solver == "vcycle" || solver == "Vcycle";
+        const bool f = solver == "fcycle" || solver == "Fcycle";
+        if (!(v || f)) throw std::invalid_argument("RefinementSolver: solver must be \"Vcycle\" or \"Fcycle\"");
+        (void)ofile;
+        void* st = outer->get_stream();
+        mgpx::StreamScope scope(st);
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        Domain<float>& d32 = inner->get_domain(0, 0);
+        const double fnorm = d64.f->infinity_norm();
+        double norm = level0_norm(d64, st) / fnorm;
+        rel_res = norm;
+        if (is_verbose) std::cout << std::setw(4) << 0 << ": Initial residual: " << std::setw(8) << rel_res << std::endl;
+        for (iter = 0; iter < settings.maxiter; iter++) {
+            // correction equation: f32 = r, e = 0, one FP32 cycle
+            const mgp_array r64 = d64.r->desc(), f32 = d32.f->desc(), e32 = d32.u->desc(), u64 = d64.u->desc();
+            mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
+            d32.u->init_zero();
+            inner->cycle(f ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
+            // u += e, then the FP64 residual and its norm
+            const mgp_array e32b = d32.u->desc();
+            (void)e32;
+            mgpx::check(mgp_add_f32_to_f64(&u64, &e32b, st), "refinement: update");
+            norm = level0_norm(d64, st) / fnorm;
+            if (is_verbose) std::cout << std::setw(4) << iter + 1 << ": Relative residual: " << std::setw(8) << norm << std::endl;
+            if (norm <= settings.tolerance) {
+                rel_res = norm;
+                break;
+            }
+            if (iter >= settings.miniter && (norm > 2.0 * rel_res || since() > maxtime)) {
+                rel_res = norm;
+                break;
+            }
+            rel_res = norm;
+        }
+        wtime = since();
+    }
+
+    double relative_residual() { return rel_res; }
+    double solve_time() { return wtime; }
+    double solve_iterations() { return (double)iter; }
+    DeviceArray<double>& get_rhs(uint_t g) { return outer->get_rhs(g); }
+    const DeviceArray<double>& get_solution(uint_t g) { return outer->get_solution(g); }
+    const DeviceArray<double>& get_residual(uint_t g) { return outer->get_residual(g); }
+    Boundary<double>& get_top_bc(uint_t g) { return outer->get_top_bc(g); }
+    Boundary<double>& get_bottom_bc(uint_t g) { return outer->get_bottom_bc(g); }
+    PoissonSolver<double, R, P, S>& fp64() { return *outer; }
+    PoissonSolver<float, R, P, S>& fp32() { return *inner; }
+    void* get_stream() const { return outer->get_stream(); }
+
+private:
+    double level0_norm(Domain<double>& d, void* st) {
+        mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
+        residual_norm<double>(d, dev_norm);
+        mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
+        mgpx::check(mgp_stream_sync(st), "refinement: norm");
+        return host_norm[0];
+    }
+
+    Settings settings, outer_settings, inner_settings;
+    BoundaryCondition BC;
+    PoissonSolver<double, R, P, S>* outer = nullptr;
+    PoissonSolver<float, R, P, S>* inner = nullptr;
+    double* dev_norm = nullptr;
+    double* host_norm = nullptr;
+    bool is_verbose = false;
+    uint_t iter = 0;
+    double rel_res = 0.0, wtime = 0.0;
+};
+}  // namespace Poisson
+#endif

@@ -52,6 +52,8 @@ public:
     bool use_graphs = true;        // replay each cycle as one captured HIP graph
     bool fused_sweeps = true;      // one-pass red-black sweeps where the colouring allows
     bool print_cycle_time = true;  // the reference prints "<cycle> took <s> seconds" per cycle
+    int_t coarse_sweeps = 0;       // CONSISTENT only: smoothing passes per visit of the coarsest
+                                   // level (0 = nsmooth, the reference's schedule)
 
     Settings() {}
     Settings(const Settings& s) = default;

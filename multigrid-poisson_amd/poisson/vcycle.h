@@ -43,8 +43,15 @@ void ascend(Domain<T>*** domains, Prolongation<T>& prolongation, uint_t level, u
 template <class T>
 void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& prolongation,
             Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10) {
+    if (level >= levels - 1) {
+        // coarsest level: smoothing only (vcycle.h:27-31); in CONSISTENT mode
+        // optionally more passes, a stronger coarsest-level solve
+        const Settings& st = domains[0][level]->settings;
+        const int_t n = (st.mode == CONSISTENT && st.coarse_sweeps > 0) ? st.coarse_sweeps : nsmooth;
+        relaxation.relax(domains, omega, n, level, num_devices);
+        return;
+    }
     relaxation.relax(domains, omega, nsmooth, level, num_devices);
-    if (level >= levels - 1) return;
     detail::descend(domains, restriction, level, num_devices);
     Vcycle<T>(domains, restriction, prolongation, relaxation, omega, level + 1, levels, num_devices, nsmooth);
     detail::ascend(domains, prolongation, level, num_devices);

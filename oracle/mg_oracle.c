@@ -600,10 +600,15 @@ static void go_up(og_solver* s, int l) {
     add_shape(&v->L, v->u, v->r);
 }
 
-/* include/vcycle.h:14-98 */
+/* include/vcycle.h:14-98 (coarsest level: nsmooth passes, or coarse_sweeps in
+ * consistent mode) */
 static void vcycle(og_solver* s, int l, double omega, int64_t nsmooth) {
+    if (l >= s->nl - 1) {
+        const int64_t n = (s->P.mode == OG_MODE_CONSISTENT && s->P.coarse_sweeps > 0) ? s->P.coarse_sweeps : nsmooth;
+        relax(s, l, omega, n);
+        return;
+    }
     relax(s, l, omega, nsmooth);
-    if (l >= s->nl - 1) return;
     go_down(s, l);
     vcycle(s, l + 1, omega, nsmooth);
     go_up(s, l);

@@ -61,6 +61,8 @@ typedef struct og_params {
     int32_t restriction;/* OG_INJECTION / OG_FULL_WEIGHTING */
     int32_t problem;    /* analytic problem id for init(); <0: init_zero() */
     int32_t threads;    /* OpenMP threads, 0 = runtime default */
+    int32_t coarse_sweeps; /* consistent mode: smoothing passes on the coarsest
+                              level per visit (0 = nsmooth, as the reference) */
 } og_params;
 
 /* ---- layout helpers ---- */

@@ -29,7 +29,7 @@ class Params(ctypes.Structure):
     _fields_ = [("dims", ctypes.c_int64 * 3), ("levels", ctypes.c_int64), ("h", ctypes.c_double),
                 ("origin", ctypes.c_double * 3), ("bc", ctypes.c_int32 * 6), ("mode", ctypes.c_int32),
                 ("relaxation", ctypes.c_int32), ("restriction", ctypes.c_int32),
-                ("problem", ctypes.c_int32), ("threads", ctypes.c_int32)]
+                ("problem", ctypes.c_int32), ("threads", ctypes.c_int32), ("coarse_sweeps", ctypes.c_int32)]
 
 
 _lib = None
@@ -117,7 +117,7 @@ class Solver:
     """One oracle solver: PoissonSolver<double, R, TrilinearInterpolation, S>."""
 
     def __init__(self, dims, levels, h, origin=(0.0, 0.0, 0.0), bc="DDDDDD", mode=REFERENCE,
-                 relaxation=GAUSS_SEIDEL, restriction=INJECTION, problem=0, threads=0):
+                 relaxation=GAUSS_SEIDEL, restriction=INJECTION, problem=0, threads=0, coarse_sweeps=0):
         p = Params()
         p.dims[:] = [int(x) for x in dims]
         p.levels = int(levels)
@@ -125,7 +125,7 @@ class Solver:
         p.origin[:] = [float(x) for x in origin]
         p.bc[:] = [NEUMANN if c == "N" else DIRICHLET for c in bc]
         p.mode, p.relaxation, p.restriction = int(mode), int(relaxation), int(restriction)
-        p.problem, p.threads = int(problem), int(threads)
+        p.problem, p.threads, p.coarse_sweeps = int(problem), int(threads), int(coarse_sweeps)
         self.params = p
         self.levels = int(levels)
         self.h = lib().og_create(ctypes.byref(p))

@@ -7,6 +7,7 @@ and off.
 """
 import ctypes
 import sys
+import zlib
 
 import numpy as np
 import pytest
@@ -291,3 +292,51 @@ def test_consistent_fcycle_reaches_1e10_at_129():
     assert rel <= 1e-10
     s.to_host()
     np.testing.assert_array_equal(s.array("u"), o.array("u"))
+
+
+SWEEP_SHAPES = [((40, 33, 130), "NNNNDD"), ((9, 17, 65), "DDDDDD"), ((33, 30, 64), "NDDNND"),
+                ((17, 15, 129), "DNNDDN"), ((65, 9, 9), "NNNNNN"), ((5, 5, 5), "DNDNDN")]
+
+
+@pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
+@pytest.mark.parametrize("shape,bc", SWEEP_SHAPES, ids=[str(i) for i in range(len(SWEEP_SHAPES))])
+def test_one_pass_sweep_equals_oracle_relax(shape, bc, mode):
+    """mgp_rbgs_sweep (tiles marching along x) == the oracle's relax() (ghost
+    refresh, red half-sweep, ghost refresh, black half-sweep), bit for bit, on
+    shapes with partial and multiple tiles in y and z and x-chunks."""
+    say("case sweep", shape, bc, mode)
+    omega = 0.93
+    halo = [1 if c == "N" else 0 for c in bc]
+    o = O.Solver(shape, 1, 1.0 / (shape[0] - 1), (0, 0, 0), bc, mode, O.GAUSS_SEIDEL, O.INJECTION, -1)
+    eff = [O.NEUMANN if o.face_type(q) == O.NEUMANN else O.DIRICHLET for q in range(6)]
+    if mode == mgp.REFERENCE:
+        L = o.layout()
+        if not (L.s[0] % 2 and L.s[1] % 2):
+            pytest.skip("reference colouring is not a checkerboard for this halo layout")
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc)).encode()))
+    L = o.layout()
+    o.array("u")[:] = rng.standard_normal(L.size)
+    o.array("f")[:] = rng.standard_normal(L.size)
+    for q in range(6):
+        o.face(q)[:] = rng.standard_normal(o.face(q).size)
+    u0 = o.array("u").copy()
+    f0 = o.array("f").copy()
+    h = o.level_h(0)
+    ui = mgp.DeviceArray(shape, halo).upload(u0)
+    uo = mgp.DeviceArray(shape, halo).upload(u0)
+    f = mgp.DeviceArray(shape, halo).upload(f0)
+    faces = {q: mgp.DeviceBuffer(o.face(q).copy()) for q in range(6)}
+    F = mgp.Faces()
+    for q in range(6):
+        F.type[q] = eff[q]
+        F.g[q] = faces[q].ptr if eff[q] == mgp.NEUMANN else None
+    F.h = h
+    box = [int(eff[O.W] == 0), shape[0] - int(eff[O.E] == 0), int(eff[O.S] == 0), shape[1] - int(eff[O.N] == 0),
+           int(eff[O.B] == 0), shape[2] - int(eff[O.T] == 0)]
+    for s in range(2):
+        mgp.call("mgp_rbgs_sweep_f64", mode, ui.ref(), uo.ref(), f.ref(), ctypes.byref(F), mgp.box(box), omega, None)
+        ui, uo = uo, ui
+    o.relax(0, omega, 2)
+    got = O.interior(L, ui.download().ravel())
+    want = O.interior(L, o.array("u"))
+    np.testing.assert_array_equal(got, want)
