@@ -13,6 +13,12 @@ A step is one complete solve to the tolerance; the right-hand side and the
 hierarchy stay resident in HBM, each step restarts from u = 0 on the device.
 value = fine-grid DOF x steps / wall time of the timed steps.
 
+The mixed-precision leg (same grid and cycle, all-Dirichlet faces,
+u = cos(x z^2) sin(y^3)) times the FP32-cycle / FP64-refinement solver and the
+plain FP64 solver on the same problem, and reports their DOF/s, the ratio, and
+the max-abs difference of the mixed solution from the FP64 one and from the
+analytic u (--no-mixed skips it).
+
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
 runs on one GPU.
 """
@@ -73,6 +79,7 @@ def main():
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
                     help="mgp_tuning_set before the run (e.g. sweep.target_wgs=4096)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32+FP64-refinement leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the solve on one GPU; --gpus %d is not supported "
@@ -94,19 +101,7 @@ def main():
                    coarse_sweeps=args.coarse_sweeps)
     s.to_device()
     log("bench: hierarchy resident; warmup")
-    for _ in range(args.warmup):
-        s.reset()
-        s.solve("Fcycle", args.nsmooth, 1.0)
-    s.sync()
-    cycles, rel = [], []
-    t0 = time.perf_counter()
-    for k in range(args.steps):
-        s.reset()
-        it, rr, _t = s.solve("Fcycle", args.nsmooth, 1.0)
-        cycles.append(it + 1)
-        rel.append(rr)
-    s.sync()
-    elapsed = time.perf_counter() - t0
+    elapsed, cycles, rel = timed_solves(s, args)
     ms_per_step = 1e3 * elapsed / args.steps
     value = dof * args.steps / elapsed
     log("bench: %.1f ms/solve, %d cycles, rel_res %.3e" % (ms_per_step, cycles[-1], rel[-1]))
@@ -151,6 +146,11 @@ def main():
     resid_gbs = resid_bytes / (resid_ms * 1e-3) / 1e9
     log("bench: sweep %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" % (sweep_ms, sweep_gbs, resid_ms, resid_gbs))
 
+    mixed = None
+    if not args.no_mixed:
+        s.close()
+        mixed = mixed_leg(dims, h, args)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads, args.coarse_sweeps)
@@ -177,9 +177,79 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
                      "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
         "kernels": {"residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
+        "fp32_refine": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def timed_solves(s, args):
+    """Warmup solves, then args.steps timed solves from u = 0; returns (seconds,
+    cycles per solve, final relative residuals)."""
+    for _ in range(args.warmup):
+        s.reset()
+        s.solve("Fcycle", args.nsmooth, 1.0)
+    s.sync()
+    cycles, rel = [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        s.reset()
+        it, rr, _t = s.solve("Fcycle", args.nsmooth, 1.0)
+        cycles.append(it + 1)
+        rel.append(rr)
+    s.sync()
+    return time.perf_counter() - t0, cycles, rel
+
+
+def interior(a, shape, halo):
+    """(nx, ny, nz) view of a halo-padded flat host array."""
+    hN, hS, hE, hW, hT, hB = halo
+    p = mgp.padded_shape(shape, halo)
+    return a.reshape(p)[hW:hW + shape[0], hS:hS + shape[1], hB:hB + shape[2]]
+
+
+def mixed_leg(dims, h, args):
+    """FP32 cycles inside FP64 refinement vs plain FP64, all-Dirichlet,
+    u = cos(x z^2) sin(y^3), same grid, levels, cycle and tolerance."""
+    import numpy as np
+
+    dof = dims[0] * dims[1] * dims[2]
+    out = {"workload": "poisson-fcycle-mixed-%dx%dx%d" % tuple(dims), "bc": "Dirichlet all faces",
+           "u": "cos(x z^2) sin(y^3)"}
+    sols = {}
+    for name, dt in (("fp64", mgp.F64), ("fp32_refine", mgp.MIXED)):
+        log("bench: mixed leg, %s solver" % name)
+        s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), "DDDDDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                       mgp.FULL_WEIGHTING, problem=3, dtype=dt,
+                       flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0), maxiter=100, miniter=1,
+                       tolerance=args.tol, coarse_sweeps=args.coarse_sweeps)
+        s.to_device()
+        el, cyc, rel = timed_solves(s, args)
+        out[name] = {"DOF/s": dof * args.steps / el, "ms_per_solve": 1e3 * el / args.steps, "cycles": cyc[-1],
+                     "final_rel_res": rel[-1]}
+        log("bench:   %.1f ms/solve, %d cycles, rel_res %.3e" % (1e3 * el / args.steps, cyc[-1], rel[-1]))
+        s.to_host()
+        host, a = s._arr(0, "u")
+        sols[name] = np.array(interior(s.array("u"), list(a.shape), list(a.halo)), copy=True)
+        s.close()
+    out["speedup"] = out["fp32_refine"]["DOF/s"] / out["fp64"]["DOF/s"]
+    a, b = sols["fp32_refine"], sols["fp64"]
+    x = np.arange(dims[0]) * h
+    y = np.arange(dims[1]) * h
+    z = np.arange(dims[2]) * h
+    sy3 = np.sin(y ** 3)[None, :, None]
+    d_fp64 = err_mixed = err_fp64 = 0.0
+    for i0 in range(0, dims[0], 64):
+        xs = x[i0:i0 + 64][:, None, None]
+        ua = np.cos(xs * (z ** 2)[None, None, :]) * sy3
+        d_fp64 = max(d_fp64, float(np.max(np.abs(a[i0:i0 + 64] - b[i0:i0 + 64]))))
+        err_mixed = max(err_mixed, float(np.max(np.abs(a[i0:i0 + 64] - ua))))
+        err_fp64 = max(err_fp64, float(np.max(np.abs(b[i0:i0 + 64] - ua))))
+    out["max_abs_diff_vs_fp64"] = d_fp64
+    out["max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp64": err_fp64}
+    log("bench: mixed %.2fx FP64, |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" %
+        (out["speedup"], d_fp64, err_mixed, err_fp64))
+    return out
 
 
 def cpu_baseline(dims, levels, h, nsmooth, cycles, threads, coarse_sweeps):

@@ -254,7 +254,9 @@ const char* mgp_last_error(void);
  * graphs recorded afterwards).  Keys: "sweep.target_wgs" (workgroups a
  * one-pass sweep aims for, default 4096), "sweep.min_chunk" (shortest x march
  * per workgroup, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup,
- * default 8).  No counterpart in the reference, whose launch
+ * default 8), "sweep.kernel" (1: workgroup tiles sharing planes through LDS,
+ * 2: one wave per tile, values in registers), "sweep.rows" (8, 12 or 16
+ * register rows per wave tile, kernel 2).  No counterpart in the reference, whose launch
  * shape was fixed by the OpenMP runtime. */
 int mgp_tuning_set(const char* key, int64_t value);
 int mgp_tuning_get(const char* key, int64_t* value);

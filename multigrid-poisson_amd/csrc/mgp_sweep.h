@@ -16,4 +16,6 @@ int64_t march_xchunk(const int64_t box[6]);
 extern int64_t g_sweep_target_wgs;
 extern int64_t g_sweep_min_chunk;
 extern int64_t g_sweep_waves;
+extern int64_t g_sweep_kernel;
+extern int64_t g_sweep_rows;
 }  // namespace mgp

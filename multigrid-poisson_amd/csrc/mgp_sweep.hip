@@ -176,9 +176,10 @@ __global__ void __launch_bounds__(64 * (RH / NQ)) k_rbgs_march(MarchArgs<T> A) {
     const int64_t byc = min(max(by, (int64_t)0), n1 - 1), bzc = min(max(bz, (int64_t)0), n2 - 1);
     const uint32_t boff = (uint32_t)((byc * s1 + bzc) * (int64_t)sizeof(T));
     const uint32_t bfoff = (uint32_t)((byc * n2 + bzc) * (int64_t)sizeof(T));
-    // Neumann ghost positions of this tile (uniform: only boundary tiles)
-    const bool yz_ghosts = (y0 == 0 && A.g[MGP_SOUTH]) || (y0 + TY >= n1 && A.g[MGP_NORTH]) ||
-                           (z0 == 0 && A.g[MGP_BOTTOM]) || (z0 + TZ >= n2 && A.g[MGP_TOP]);
+    // Neumann ghost positions in this tile's old-plane rectangle, rows
+    // y0-2 .. y0+TY+1 and columns z0-2 .. z0+TZ+1 (uniform: boundary tiles only)
+    const bool yz_ghosts = (y0 - 2 < 0 && A.g[MGP_SOUTH]) || (y0 + TY + 1 >= n1 && A.g[MGP_NORTH]) ||
+                           (z0 - 2 < 0 && A.g[MGP_BOTTOM]) || (z0 + TZ + 1 >= n2 && A.g[MGP_TOP]);
     const T* gW = A.g[MGP_WEST];
     const T* gE = A.g[MGP_EAST];
 
@@ -333,6 +334,259 @@ __global__ void __launch_bounds__(64 * (RH / NQ)) k_rbgs_march(MarchArgs<T> A) {
     }
 }
 
+
+// ---------------------------------------------------------------------------
+// Wave-private variant: one wave owns a (y, z) tile and marches along x with
+// every value in registers; no LDS and no barriers, so each wave streams at
+// its own pace and keeps a whole plane of loads in flight.
+//
+// Lane l holds column z = z0 - 2 + l; register row t (0 .. R+1) holds
+// y = y0 - 2 + t.  Red points are updated on lanes 1..62 and rows 1..R (the
+// tile plus a one-point ring), black points on lanes 2..61 and rows 2..R-1
+// (the tile, WZ x (R-2) points), which are the ones written.  z-neighbours come
+// from the adjacent lanes through DPP wave shifts, y-neighbours from the
+// adjacent register rows, x-neighbours from the previous and next planes'
+// registers.  A plane's registers hold the old values until its red points
+// are updated in place, then the new red values (black points keep the old
+// value until their own update one step later), which is exactly what each
+// half-sweep's stencil reads.
+constexpr int WZ = 60;  // interior columns per wave tile
+
+template <class T>
+__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1 (z - 1)
+    return __builtin_amdgcn_update_dpp((T)0, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+}
+template <class T>
+__device__ __forceinline__ T lane_next(T v) {  // value of lane l+1 (z + 1)
+    return __builtin_amdgcn_update_dpp((T)0, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+}
+
+// Buffer-resource access (SGPR descriptor + 32-bit lane offset + SGPR row
+// offset): no per-lane address arithmetic.  A lane offset at or beyond the
+// record count drops the store in hardware, which masks stores without
+// branches; only the lane (VGPR) offset is used for that, since the range
+// check does not cover the SGPR offset.  Loads always use in-range offsets.
+typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+constexpr uint32_t BUF_OOB = 0x40000000u;  // offset past any plane (planes < 1 GiB)
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
+template <>
+__device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+template <>
+__device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+}
+
+// EDGE: the tile has rows outside the box or in-plane Neumann ghost positions
+// (boundary tiles only); the other tiles run a copy of the loop without the
+// row masks and the ghost code
+template <class T, int MODE, int R, bool EDGE>
+__device__ __forceinline__ void wave_march(const MarchArgs<T>& A, int tl) {
+    constexpr int NR = R + 2;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * (R - 2);
+    const int64_t z0 = A.bz0 + (int64_t)tz * WZ;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+    const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2, s0 = A.ui.s0, s1 = A.ui.s1;
+    const T two_h = A.two_h;
+    // bytes from a plane's (y, z) = (0, 0) to the end of its last row
+    const uint32_t plane_bytes = (uint32_t)(((n1 - 1) * s1 + n2) * (int64_t)sizeof(T));
+
+    const int lane = threadIdx.x;
+    const int64_t z = z0 - 2 + lane;
+    const bool z_in = z >= A.bz0 && z < A.bz1;
+    const bool z_shape = z >= 0 && z < n2;
+    const bool lane_red = z_in && lane >= 1 && lane <= 62;
+    const bool lane_core = z_in && lane >= 2 && lane <= 61;
+    const int64_t zc = min(max(z, (int64_t)0), n2 - 1);
+    const uint32_t zoff = (uint32_t)(zc * (int64_t)sizeof(T));
+    const uint32_t zoff_st = lane_core ? zoff : BUF_OOB;
+    // colour: red where (i + y + z) is even (consistent), or where the
+    // halo-inclusive linear index is even (reference, include/gaussseidel.h:56,81;
+    // odd strides make that the parity of lin0 + i + y + z)
+    const int cbase = (int)(((MODE == MGP_MODE_REFERENCE ? A.lin0 : 0) + z + y0) & 1);
+
+    // row byte offsets, clamped into the shape, and the rows inside the box
+    uint32_t roff[NR];
+    bool ybox[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int64_t y = y0 - 2 + t;
+        roff[t] = (uint32_t)(min(max(y, (int64_t)0), n1 - 1) * s1 * (int64_t)sizeof(T));
+        ybox[t] = y >= A.by0 && y < A.by1;
+    }
+
+    const T* gW = A.g[MGP_WEST];
+    const T* gE = A.g[MGP_EAST];
+    const T* gS = A.g[MGP_SOUTH];
+    const T* gN = A.g[MGP_NORTH];
+    const T* gB = A.g[MGP_BOTTOM];
+    const T* gT = A.g[MGP_TOP];
+    const bool y_ghosts = (y0 - 2 < 0 && gS) || (y0 + R - 1 >= n1 && gN);
+    const bool z_ghosts = (z0 - 2 < 0 && gB) || (z0 + WZ + 1 >= n2 && gT);
+
+    // old values of plane j (rows 0..R+1); planes -1 and n0 are the Neumann
+    // x-ghosts sign*2h*g + mirror plane (include/neumann.h:123-124), formed
+    // from the old mirror plane
+    auto load_plane = [&](int64_t j, T (&P)[NR]) {
+        const int64_t jm = min(max(j < 0 ? (int64_t)1 : (j >= n0 ? n0 - 2 : j), (int64_t)0), n0 - 1);
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.ui.p + jm * s0, plane_bytes);
+#pragma unroll
+        for (int t = 0; t < NR; t++) P[t] = bload<T>(r, zoff, roff[t]);
+        if ((j < 0 && gW) || (j >= n0 && gE)) {
+            const T* g = j < 0 ? gW : gE;
+            const T sg = j < 0 ? (T)-1.0 : (T)1.0;
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = min(max(y0 - 2 + t, (int64_t)0), n1 - 1);
+                P[t] = sg * two_h * g[y * n2 + zc] + P[t];  // face (1, ny, nz)
+            }
+        }
+    };
+    auto load_f = [&](int64_t j, T (&F)[NR]) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.f.p + min(max(j, (int64_t)0), n0 - 1) * s0, plane_bytes);
+#pragma unroll
+        for (int t = 1; t <= R; t++) F[t] = bload<T>(r, zoff, roff[t]);
+    };
+    // in-plane Neumann ghosts of plane i from the values in P (the ghost
+    // refresh before a half-sweep): y ghosts from the rows two apart, z ghosts
+    // from the lanes two apart
+    auto fix_ghosts = [&](int64_t i, T (&P)[NR]) {
+        if (!EDGE) return;
+        if (y_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 2 + t;
+                if (t + 2 < NR && y == -1 && gS) {
+                    const T v = (T)-1.0 * two_h * gS[i * n2 + zc] + P[t + 2];  // face (nx, 1, nz)
+                    P[t] = z_shape ? v : P[t];
+                } else if (t >= 2 && y == n1 && gN) {
+                    const T v = (T)1.0 * two_h * gN[i * n2 + zc] + P[t - 2];
+                    P[t] = z_shape ? v : P[t];
+                }
+            }
+        }
+        if (z_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 2 + t;
+                if (y < 0 || y >= n1) continue;
+                if (gB) {
+                    const T m = lane_next(lane_next(P[t]));
+                    const T v = (T)-1.0 * two_h * gB[i * n1 + y] + m;  // face (nx, ny, 1)
+                    P[t] = z == -1 ? v : P[t];
+                }
+                if (gT) {
+                    const T m = lane_prev(lane_prev(P[t]));
+                    const T v = (T)1.0 * two_h * gT[i * n1 + y] + m;
+                    P[t] = z == n2 ? v : P[t];
+                }
+            }
+        }
+    };
+
+    T Pm2[NR], Pm1[NR], P0[NR], Pp1[NR], Pp2[NR], Fm1[NR], F0[NR], Fp1[NR];
+    const int64_t i0 = xa - 1;
+    load_plane(i0 - 1, Pm1);
+    load_plane(i0, P0);
+    load_plane(i0 + 1, Pp1);
+    load_f(i0, F0);
+#pragma unroll
+    for (int t = 0; t < NR; t++) Pm2[t] = Fm1[t] = (T)0;
+
+    for (int64_t i = i0; i <= xb; i++) {
+        // next planes in flight during this step
+        load_plane(i + 2, Pp2);
+        load_f(i + 1, Fp1);
+        const bool plane_in = i >= 0 && i < n0;  // uniform
+        const bool x_in = i >= A.bx0 && i < A.bx1;
+        const int par = (int)(i & 1);
+        if (plane_in) fix_ghosts(i, P0);
+        // red points of plane i
+        if (x_in) {
+#pragma unroll
+            for (int t = 1; t <= R; t++) {
+                const bool red = lane_red && (!EDGE || ybox[t]) && ((cbase ^ par ^ (t & 1)) == 0);
+                const T r = update<T, MODE>(P0[t], Pm1[t], Pp1[t], P0[t - 1], P0[t + 1], lane_prev(P0[t]),
+                                            lane_next(P0[t]), F0[t], A.one_omega, A.omega_sixth, A.hsq);
+                P0[t] = red ? r : P0[t];
+            }
+        }
+        if (plane_in) fix_ghosts(i, P0);
+        // black points of plane p = i-1, then both colours of plane p out
+        const int64_t p = i - 1;
+        if (p >= xa && p < xb) {
+            const int pb = par ^ 1;
+            const __amdgpu_buffer_rsrc_t ro = plane_rsrc(A.uo.p + p * s0, plane_bytes);
+            // x-ghost planes beyond Neumann x faces, from the new red values of
+            // the mirror plane (the ghost refresh before the black half-sweep);
+            // planes -1 and n0 are not needed after this step (n0 >= 3)
+            if (p == 0 && gW) {
+#pragma unroll
+                for (int t = 2; t < R; t++) {
+                    const int64_t yc = min(max(y0 - 2 + t, (int64_t)0), n1 - 1);
+                    Pm2[t] = (T)-1.0 * two_h * gW[yc * n2 + zc] + P0[t];
+                }
+            }
+            if (p == n0 - 1 && gE) {
+#pragma unroll
+                for (int t = 2; t < R; t++) {
+                    const int64_t yc = min(max(y0 - 2 + t, (int64_t)0), n1 - 1);
+                    P0[t] = (T)1.0 * two_h * gE[yc * n2 + zc] + Pm2[t];
+                }
+            }
+#pragma unroll
+            for (int t = 2; t < R; t++) {
+                const bool black = lane_core && ((cbase ^ pb ^ (t & 1)) == 1);
+                const T b = update<T, MODE>(Pm1[t], Pm2[t], P0[t], Pm1[t - 1], Pm1[t + 1], lane_prev(Pm1[t]),
+                                            lane_next(Pm1[t]), Fm1[t], A.one_omega, A.omega_sixth, A.hsq);
+                Pm1[t] = black ? b : Pm1[t];
+                bstore(Pm1[t], ro, !EDGE || ybox[t] ? zoff_st : BUF_OOB, roff[t]);
+            }
+        }
+        // rotate the x window
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            Pm2[t] = Pm1[t];
+            Pm1[t] = P0[t];
+            P0[t] = Pp1[t];
+            Pp1[t] = Pp2[t];
+            Fm1[t] = F0[t];
+            F0[t] = Fp1[t];
+        }
+    }
+}
+
+template <class T, int MODE, int R>
+__global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int ty = (tl / A.ztiles) % A.ytiles, tz = tl % A.ztiles;
+    const int64_t y0 = A.by0 + (int64_t)ty * (R - 2), z0 = A.bz0 + (int64_t)tz * WZ;
+    // red rows y0-1 .. y0+R-2 all inside the box, and no ghost positions
+    const bool edge = y0 - 1 < A.by0 || y0 + R - 2 >= A.by1 || (y0 - 2 < 0 && A.g[MGP_SOUTH]) ||
+                      (y0 + R - 1 >= A.ui.n1 && A.g[MGP_NORTH]) || (z0 - 2 < 0 && A.g[MGP_BOTTOM]) ||
+                      (z0 + WZ + 1 >= A.ui.n2 && A.g[MGP_TOP]);
+    if (edge) wave_march<T, MODE, R, true>(A, tl);
+    else wave_march<T, MODE, R, false>(A, tl);
+}
+
 }  // namespace
 
 template <class T>
@@ -359,8 +613,32 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
     A.ntiles = A.ytiles * A.ztiles * xchunks;
     A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
     const hipStream_t st = (hipStream_t)stream;
-    const dim3 g(A.ntiles_pad);
     const bool ref = mode == MGP_MODE_REFERENCE;
+    // wave tiles: planes addressed through buffer descriptors with a 32-bit
+    // range (a plane must stay below the out-of-range lane offset)
+    const int64_t plane_bytes = ((A.ui.n1 - 1) * A.ui.s1 + A.ui.n2) * (int64_t)sizeof(T);
+    if (g_sweep_kernel == 2 && A.ui.n0 >= 3 && plane_bytes < (int64_t)BUF_OOB) {
+        const int rows = g_sweep_rows == 8 || g_sweep_rows == 12 ? (int)g_sweep_rows : 16;
+        A.ytiles = (int)((b[3] - b[2] + rows - 3) / (rows - 2));
+        A.ztiles = (int)((b[5] - b[4] + WZ - 1) / WZ);
+        const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+        int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        A.xchunk = (nx + chunks - 1) / chunks;
+        A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
+        A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+        const dim3 g(A.ntiles_pad);
+#define MGP_WAVE_LAUNCH(RR)                                                                          \
+    if (ref) hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_REFERENCE, RR>), g, dim3(64), 0, st, A); \
+    else hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_CONSISTENT, RR>), g, dim3(64), 0, st, A);
+        if (rows == 8) { MGP_WAVE_LAUNCH(8) }
+        else if (rows == 12) { MGP_WAVE_LAUNCH(12) }
+        else { MGP_WAVE_LAUNCH(16) }
+#undef MGP_WAVE_LAUNCH
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? MGP_OK : (int)e;
+    }
+    const dim3 g(A.ntiles_pad);
     switch (g_sweep_waves) {
     case 4:
         if (ref) hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_REFERENCE, 4>), g, dim3(256), 0, st, A);
@@ -383,6 +661,8 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
 int64_t g_sweep_target_wgs = 4096;  // workgroups to aim for per launch
 int64_t g_sweep_min_chunk = 48;     // shortest x march per workgroup
 int64_t g_sweep_waves = 8;          // waves per workgroup: 4, 8 or 16
+int64_t g_sweep_kernel = 1;         // 1: workgroup tiles with LDS, 2: wave-private tiles
+int64_t g_sweep_rows = 16;          // register rows per wave tile: 8, 12 or 16
 
 // x-chunk length: enough workgroups to keep every CU busy with several
 // marches, but chunks long enough that the two warm-up planes per chunk stay

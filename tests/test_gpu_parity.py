@@ -295,16 +295,39 @@ def test_consistent_fcycle_reaches_1e10_at_129():
 
 
 SWEEP_SHAPES = [((40, 33, 130), "NNNNDD"), ((9, 17, 65), "DDDDDD"), ((33, 30, 64), "NDDNND"),
-                ((17, 15, 129), "DNNDDN"), ((65, 9, 9), "NNNNNN"), ((5, 5, 5), "DNDNDN")]
+                ((17, 15, 129), "DNNDDN"), ((65, 9, 9), "NNNNNN"), ((5, 5, 5), "DNDNDN"),
+                # tile borders exactly on the north and top ghost planes
+                ((9, 29, 125), "NNNNNN")]
 
 
+SWEEP_KERNELS = [(1, 16), (2, 8), (2, 12), (2, 16)]  # (sweep.kernel, sweep.rows)
+
+
+@pytest.fixture
+def sweep_kernel(request):
+    k, rows = request.param
+    L = mgp.lib()
+    old = {}
+    for key in ("sweep.kernel", "sweep.rows"):
+        v = ctypes.c_int64()
+        mgp.check(L.mgp_tuning_get(key.encode(), ctypes.byref(v)))
+        old[key] = v.value
+    mgp.check(L.mgp_tuning_set(b"sweep.kernel", k))
+    mgp.check(L.mgp_tuning_set(b"sweep.rows", rows))
+    yield k, rows
+    for key, v in old.items():
+        mgp.check(L.mgp_tuning_set(key.encode(), v))
+
+
+@pytest.mark.parametrize("sweep_kernel", SWEEP_KERNELS, ids=["k%d_r%d" % kr for kr in SWEEP_KERNELS], indirect=True)
 @pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
 @pytest.mark.parametrize("shape,bc", SWEEP_SHAPES, ids=[str(i) for i in range(len(SWEEP_SHAPES))])
-def test_one_pass_sweep_equals_oracle_relax(shape, bc, mode):
-    """mgp_rbgs_sweep (tiles marching along x) == the oracle's relax() (ghost
-    refresh, red half-sweep, ghost refresh, black half-sweep), bit for bit, on
-    shapes with partial and multiple tiles in y and z and x-chunks."""
-    say("case sweep", shape, bc, mode)
+def test_one_pass_sweep_equals_oracle_relax(shape, bc, mode, sweep_kernel):
+    """mgp_rbgs_sweep (tiles marching along x: workgroup tiles through LDS, or
+    one wave per tile in registers) == the oracle's relax() (ghost refresh, red
+    half-sweep, ghost refresh, black half-sweep), bit for bit, on shapes with
+    partial and multiple tiles in y and z and x-chunks."""
+    say("case sweep", shape, bc, mode, sweep_kernel)
     omega = 0.93
     halo = [1 if c == "N" else 0 for c in bc]
     o = O.Solver(shape, 1, 1.0 / (shape[0] - 1), (0, 0, 0), bc, mode, O.GAUSS_SEIDEL, O.INJECTION, -1)
@@ -340,3 +363,40 @@ def test_one_pass_sweep_equals_oracle_relax(shape, bc, mode):
     got = O.interior(L, ui.download().ravel())
     want = O.interior(L, o.array("u"))
     np.testing.assert_array_equal(got, want)
+
+
+MIXED_CASES = [((65, 65, 33), 4, "NNNNDD", 2, "Fcycle"), ((33, 33, 33), 3, "DDDDDD", 0, "Vcycle"),
+               ((129, 129, 65), 5, "NNNNDD", 2, "Fcycle")]
+
+
+@pytest.mark.parametrize("case", MIXED_CASES, ids=[str(i) for i in range(len(MIXED_CASES))])
+def test_mixed_precision_refinement_reaches_fp64_solution(case):
+    """FP32 cycles inside FP64 iterative refinement (dtype MGP_MIXED) reach
+    ||r||/||f|| <= 1e-10 of the FP64 operator, and the solution agrees with the
+    oracle's FP64 solve to the same tolerance: |u_mixed - u_oracle| <= 1e-8
+    (both are within ~1e-10 |f| / lambda_min of the one discrete solution)."""
+    dims, levels, bc, prob, cyc = case
+    say("case mixed", case)
+    h = 1.0 / (dims[0] - 1)
+    s = mgp.Solver(dims, levels, h, (0, 0, 0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL, mgp.FULL_WEIGHTING, prob,
+                   dtype=mgp.MIXED, maxiter=60, miniter=1, tolerance=1e-10, coarse_sweeps=16)
+    s.to_device()
+    it, rel, _t = s.solve(cyc, 2, 1.0)
+    assert rel <= 1e-10, (it, rel)
+    o = O.Solver(dims, levels, h, (0, 0, 0), bc, O.CONSISTENT, O.GAUSS_SEIDEL, O.FULL_WEIGHTING, prob,
+                 coarse_sweeps=16)
+    o.to_device()
+    oit, orel, _ = o.solve(O.cycle_id(cyc), 2, 1.0, maxiter=60, miniter=1, tol=1e-10)
+    assert orel <= 1e-10
+    # FP32 correction cycles contract like FP64 ones until FP32 rounding (~1e-7
+    # of the residual) dominates, so the count stays close to FP64's
+    assert it <= oit + 3, (it, oit)
+    s.to_host()
+    L = o.layout()
+    a = O.interior(L, s.array("u").ravel())
+    b = O.interior(L, o.array("u"))
+    assert np.max(np.abs(a - b)) <= 1e-8 * max(1.0, np.max(np.abs(b)))
+    # a second solve from u = 0 repeats the first exactly (same launches)
+    s.reset()
+    it2, rel2, _ = s.solve(cyc, 2, 1.0)
+    assert (it2, rel2) == (it, rel)
