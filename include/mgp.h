@@ -246,8 +246,102 @@ int mgp_solver_array(void* solver, int level, int which, void** host, mgp_array*
 int mgp_solver_face(void* solver, int level, int loc, void** host, mgp_array* device, int* type);
 /* VTK of level 0's u (save, libpoisson.h:359-363) */
 int mgp_solver_save(void* solver, const char* path);
-/* message of the last failing mgp_solver_* call on this thread */
+/* message of the last failing mgp_solver_* / mgp_batch_* call on this thread */
 const char* mgp_last_error(void);
+
+/* synthetic right-hand side over the shape of f: sum over nmodes (<= 16) of
+ * amp[m] sin(a pi x) sin(b pi y) sin(c pi z), (a, b, c) = abc[3m..3m+2], at
+ * x = i h, y = j h, z = k h.  No counterpart in the reference (its f comes
+ * from compile-time analytic functions). */
+#define MGP_MAX_MODES 16
+int mgp_fill_modes_f64(const mgp_array* f, int64_t nmodes, const int32_t* abc, const double* amp, double h,
+                       void* stream);
+
+/* ---------------------------------------------------------------- batched solves
+ * Many independent problems of uneven shapes solved together: zero Dirichlet
+ * data on every face, h = 1/(nx-1), FP32 V-cycles (full weighting, trilinear
+ * prolongation, the chosen smoother) inside FP64 iterative refinement on
+ * ||r||inf/||f||inf of the FP64 residual, one launch per operation and level
+ * for all problems, a converged problem drops out of later launches.  Each
+ * problem's result is bit-identical to a single solver created with
+ * dtype MGP_MIXED, mode MGP_MODE_CONSISTENT, bc all MGP_DIRICHLET, restriction
+ * MGP_FULL_WEIGHTING and the same levels/smoother/tolerances, solved with
+ * "Vcycle".  No counterpart in the reference (one problem per PoissonSolver).
+ * Tile shapes per level size are chosen by timing candidates at creation and
+ * cached per process. */
+#define MGP_BATCH_NO_GRAPHS 1 /* launch kernels one by one (no recorded iteration) */
+#define MGP_BATCH_RETUNE 2    /* time the tile candidates even when cached */
+typedef struct {
+    int64_t count;         /* problems */
+    int64_t levels;        /* levels per problem; 0: down to a 3-point coarsest axis */
+    int32_t relaxation;    /* MGP_JACOBI or MGP_GAUSS_SEIDEL */
+    int32_t flags;         /* MGP_BATCH_* */
+    int64_t nsmooth;       /* pre- and post-smoothing passes */
+    double omega;          /* <= 0: 6/7 (Jacobi) or 8/9 (Gauss-Seidel), as the reference's defaults */
+    int64_t coarse_sweeps; /* passes on a problem's coarsest level; 0: nsmooth */
+    int64_t maxiter, miniter;
+    double tolerance;
+} mgp_batch_config;
+/* dims: count x 3 axis sizes (each 2^k m + 1 with the levels' coarsening) */
+int mgp_batch_create(const mgp_batch_config* config, const int64_t* dims, void** batch);
+int mgp_batch_destroy(void* batch);
+/* host f of problem p over its whole shape, C order (x slowest) */
+int mgp_batch_set_rhs(void* batch, int64_t p, const double* f);
+/* solve every problem from u = 0; *seconds (optional) = wall time */
+int mgp_batch_solve(void* batch, double* seconds);
+int mgp_batch_stats(void* batch, int64_t p, double* rel_res, int64_t* iterations);
+/* host copy of problem p's FP64 solution over its shape */
+int mgp_batch_solution(void* batch, int64_t p, double* out);
+/* tile chosen by the autotuner for problem p's level: planes, rows per item */
+int mgp_batch_tile(void* batch, int64_t p, int64_t level, int64_t out[2]);
+
+/* ---------------------------------------------------------------- serving
+ * A request loop on one GPU: solves of mixed shapes and face types, each with
+ * zero boundary data and a Fourier-mode right-hand side (mgp_fill_modes_f64),
+ * accepted as they arrive (mgp_server_submit returns at once) and run by a
+ * worker thread as FP32 F-cycles inside FP64 iterative refinement (consistent
+ * mode, full weighting, levels down to a 3-point coarsest axis).  Requests are
+ * admitted in arrival order while fewer than max_active run and their
+ * hierarchy fits in hbm_budget next to the active and resident ones;
+ * hierarchies stay resident for later requests of the same shape and face
+ * types (least recently used evicted first); each active solve has its own
+ * stream and the worker advances every solve one refinement iteration at a
+ * time, so their work overlaps on the device.  A result is the same bits
+ * whether its request ran alone or among others.  All-Neumann requests are
+ * rejected (singular).  No counterpart in the reference. */
+typedef struct {
+    int64_t hbm_budget;     /* bytes of device memory for active + resident hierarchies */
+    int32_t max_active;     /* solves in flight at once (one stream each) */
+    int32_t relaxation;     /* MGP_GAUSS_SEIDEL or MGP_JACOBI */
+    int64_t nsmooth;        /* smoothing passes (pre = post) */
+    double omega;           /* <= 0: the smoother's default */
+    int64_t coarse_sweeps;  /* passes on the coarsest level; 0: nsmooth */
+    int64_t maxiter;        /* refinement iterations at most */
+    double tolerance;       /* on ||r||inf/||f||inf of the FP64 residual */
+} mgp_server_config;
+typedef struct {
+    int64_t dims[3];                   /* 2^k m + 1 per axis; h = 1/(dims[0]-1) */
+    int32_t bc[6];                     /* MGP_DIRICHLET / MGP_NEUMANN, order N S E W T B */
+    int32_t nmodes;                    /* right-hand side: nmodes (<= 16) Fourier modes */
+    int32_t abc[3 * MGP_MAX_MODES];
+    double amp[MGP_MAX_MODES];
+    double* u_out;                     /* host, dims[0]*dims[1]*dims[2] doubles (C order), or NULL */
+} mgp_request;
+typedef struct {
+    int32_t status;                    /* MGP_OK or an error code */
+    int64_t iterations;
+    double rel_res;
+    double latency_s;                  /* submit -> result on the host */
+    double queue_s;                    /* submit -> admission */
+    double run_s;                      /* admission -> result on the host */
+} mgp_result;
+int mgp_server_create(const mgp_server_config* config, void** server);
+/* waits for every submitted request to finish */
+int mgp_server_destroy(void* server);
+int mgp_server_submit(void* server, const mgp_request* request, int64_t* id);
+/* blocks until request `id` is done; each id can be waited for once */
+int mgp_server_wait(void* server, int64_t id, mgp_result* result);
+int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int64_t* bytes_resident);
 
 /* ---------------------------------------------------------------- tuning
  * Launch-shape knobs (process-wide; take effect for the next launches and for
@@ -256,7 +350,8 @@ const char* mgp_last_error(void);
  * per workgroup, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup,
  * default 8), "sweep.kernel" (1: workgroup tiles sharing planes through LDS,
  * 2: one wave per tile, values in registers), "sweep.rows" (8, 12 or 16
- * register rows per wave tile, kernel 2).  No counterpart in the reference, whose launch
+ * register rows per wave tile, kernel 2), "sweep.depth" (1 or 2 planes of
+ * loads in flight per wave, kernel 2).  No counterpart in the reference, whose launch
  * shape was fixed by the OpenMP runtime. */
 int mgp_tuning_set(const char* key, int64_t value);
 int mgp_tuning_get(const char* key, int64_t* value);

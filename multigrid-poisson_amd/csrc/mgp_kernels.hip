@@ -14,6 +14,7 @@
 
 #include "mgp.h"
 #include "mgp_device.h"
+#include "mgp_point.h"
 #include "mgp_sweep.h"
 
 namespace mgp {
@@ -133,6 +134,25 @@ __global__ void k_add_f32_to_f64(View<double> u, View<float> e, Box bx, int64_t 
     u.at(i, j, k) += (double)e.at(i, j, k);
 }
 
+// synthetic right-hand side: f(i,j,k) = sum_m A_m sin(a_m pi x) sin(b_m pi y)
+// sin(c_m pi z) at x = i h, y = j h, z = k h, over the shape
+struct Modes {
+    int32_t n;
+    int32_t abc[3 * MGP_MAX_MODES];
+    double amp[MGP_MAX_MODES];
+};
+
+template <class T>
+__global__ void k_fill_modes(View<T> f, Box bx, int64_t zc, Modes m, double h) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const double x = (double)i * h, y = (double)j * h, z = (double)k * h;
+    double s = 0.0;
+    for (int q = 0; q < m.n; q++)
+        s += m.amp[q] * sin(m.abc[3 * q] * M_PI * x) * sin(m.abc[3 * q + 1] * M_PI * y) * sin(m.abc[3 * q + 2] * M_PI * z);
+    f.at(i, j, k) = (T)s;
+}
+
 __global__ void k_zero_scalars(double* p, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
@@ -149,20 +169,6 @@ __device__ __forceinline__ int colour_of(int64_t lin0, int64_t s0, int64_t s1, i
     return (int)((i + j + k) & 1);
 }
 
-// One smoother update of point c from the current values around it.
-// Reference: include/gaussseidel.h:58-60 / jacobi.h:66-69
-//   (1-w) v + w/6 (vW + vE + vS + vN + vB + vT - h^2 f)
-// Consistent: v + w/6 * fma(-h^2, f, sum(v_nb - v))
-template <class T, int MODE>
-__device__ __forceinline__ T relax_point(T vc, T vW, T vE, T vS, T vN, T vB, T vT, T fc, T one_omega, T omega_sixth,
-                                         T hsq) {
-    if (MODE == MGP_MODE_REFERENCE) {
-        return one_omega * vc + omega_sixth * (vW + vE + vS + vN + vB + vT - hsq * fc);
-    } else {
-        const T s = ((vW - vc) + (vE - vc)) + ((vS - vc) + (vN - vc)) + ((vB - vc) + (vT - vc));
-        return fma(omega_sixth, fma(-hsq, fc, s), vc);
-    }
-}
 
 // include/gaussseidel.h:28-94: one half-sweep in place, neighbours read from
 // memory (the halo holds the ghosts written just before)
@@ -193,21 +199,6 @@ __global__ void k_jacobi(View<T> u, View<T> v, View<T> f, Box bx, int64_t zc, T 
 
 // ------------------------------------------------------------- residual
 
-// include/residual.h:58-79.  Reference: r = f - (sum_nb - 6.0*u)/hsq with the
-// literal 6.0 a double, as in the reference.  Consistent:
-// r = fma(-1/h^2, sum(u_nb - u), f).
-template <class T, int MODE>
-__device__ __forceinline__ T residual_point(const View<T>& u, int64_t c, T fc, T hsq, T inv_h2) {
-    const T* p = u.p;
-    if (MODE == MGP_MODE_REFERENCE) {
-        return fc - (p[c - u.s0] + p[c + u.s0] + p[c - u.s1] + p[c + u.s1] + p[c - 1] + p[c + 1] - 6.0 * p[c]) / hsq;
-    } else {
-        const T uc = p[c];
-        const T s = ((p[c - u.s0] - uc) + (p[c + u.s0] - uc)) + ((p[c - u.s1] - uc) + (p[c + u.s1] - uc)) +
-                    ((p[c - 1] - uc) + (p[c + 1] - uc));
-        return fma(-inv_h2, s, fc);
-    }
-}
 
 template <class T, int MODE>
 __global__ void k_residual(View<T> u, View<T> f, View<T> r, Box bx, int64_t zc, T hsq, T inv_h2) {
@@ -249,57 +240,14 @@ __global__ void k_inject(View<T> in, View<T> out, Box bx, int64_t zc) {
     out.at(i, j, k) = in.at(2 * i, 2 * j, 2 * k);
 }
 
-// include/fullweighting.h:59-121: neighbours clamped to [0, n-1]; the three
-// z-planes summed in the reference's term order (T tmp = plane; tmp += plane)
+// include/fullweighting.h:59-121
 template <class T>
 __global__ void k_fw(View<T> in, View<T> out, Box bx, int64_t zc) {
     int64_t i, j, k;
     if (!box_point(bx, zc, i, j, k)) return;
-    const int64_t n0 = in.n0, n1 = in.n1, n2 = in.n2;
-    const int64_t ii = min(n0 - 1, 2 * i), jj = min(n1 - 1, 2 * j), kk = min(n2 - 1, 2 * k);
-    const int64_t im = max((int64_t)0, ii - 1), ip = min(n0 - 1, ii + 1);
-    const int64_t jm = max((int64_t)0, jj - 1), jp = min(n1 - 1, jj + 1);
-    const int64_t km = max((int64_t)0, kk - 1), kp = min(n2 - 1, kk + 1);
-#define A(a, b, c) in.at(a, b, c)
-    T tmp = 1.0 * A(im, jm, km) + 2.0 * A(im, jj, km) + 1.0 * A(im, jp, km) + 2.0 * A(ii, jm, km) +
-            4.0 * A(ii, jj, km) + 2.0 * A(ii, jp, km) + 1.0 * A(ip, jm, km) + 2.0 * A(ip, jj, km) +
-            1.0 * A(ip, jp, km);
-    tmp += 2.0 * A(im, jm, kk) + 4.0 * A(im, jj, kk) + 2.0 * A(im, jp, kk) + 4.0 * A(ii, jm, kk) +
-           8.0 * A(ii, jj, kk) + 4.0 * A(ii, jp, kk) + 2.0 * A(ip, jm, kk) + 4.0 * A(ip, jj, kk) +
-           2.0 * A(ip, jp, kk);
-    tmp += 1.0 * A(im, jm, kp) + 2.0 * A(im, jj, kp) + 1.0 * A(im, jp, kp) + 2.0 * A(ii, jm, kp) +
-           4.0 * A(ii, jj, kp) + 2.0 * A(ii, jp, kp) + 1.0 * A(ip, jm, kp) + 2.0 * A(ip, jj, kp) +
-           1.0 * A(ip, jp, kp);
-#undef A
-    out.at(i, j, k) = tmp / 64.0;
+    out.at(i, j, k) = fw_point<T>(in, i, j, k);
 }
 
-// Trilinear value at fine point (i,j,k), composed exactly as the reference's
-// four passes compose it (include/trilinearinterpolation.h:50-133): injection
-// at even points, then x-, y- and z-midpoints, each 0.5*(left + right) of the
-// previous passes' values.
-template <class T>
-__device__ __forceinline__ T half_sum(T a, T b) {
-    return 0.5 * (a + b);
-}
-
-template <class T>
-__device__ __forceinline__ T tri_xy(const View<T>& c, int64_t a, int64_t b, int64_t cc, int pi, int pj) {
-    if (!pi && !pj) return c.at(a, b, cc);
-    if (pi && !pj) return half_sum<T>(c.at(a, b, cc), c.at(a + 1, b, cc));
-    if (!pi && pj) return half_sum<T>(c.at(a, b, cc), c.at(a, b + 1, cc));
-    return half_sum<T>(half_sum<T>(c.at(a, b, cc), c.at(a + 1, b, cc)),
-                       half_sum<T>(c.at(a, b + 1, cc), c.at(a + 1, b + 1, cc)));
-}
-
-template <class T>
-__device__ __forceinline__ T trilinear(const View<T>& c, int64_t i, int64_t j, int64_t k) {
-    const int pi = (int)(i & 1), pj = (int)(j & 1), pk = (int)(k & 1);
-    const int64_t a = i >> 1, b = j >> 1, cc = k >> 1;
-    const T lo = tri_xy<T>(c, a, b, cc, pi, pj);
-    if (!pk) return lo;
-    return half_sum<T>(lo, tri_xy<T>(c, a, b, cc + 1, pi, pj));
-}
 
 template <class T, bool ADD>
 __global__ void k_prolong(View<T> coarse, View<T> fine, Box bx, int64_t zc) {
@@ -476,16 +424,6 @@ static int norm_inf(const mgp_array* a, double* out, void* stream) {
 
 // constants exactly as the reference forms them from T omega and double h
 // (include/gaussseidel.h:50-52, jacobi.h:42-44)
-template <class T>
-struct RelaxConsts {
-    T one_omega, omega_sixth, hsq;
-    RelaxConsts(double omega_in, double h) {
-        const T omega = (T)omega_in;
-        one_omega = (T)(1.0 - omega);
-        omega_sixth = (T)(omega / 6.0);
-        hsq = (T)(h * h);
-    }
-};
 
 template <class T>
 static int rbgs_half(int mode, const mgp_array* u, const mgp_array* f, const int64_t b[6], double omega, double h,
@@ -730,6 +668,23 @@ int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* s) {
     const BoxGrid g = box_grid(bx);
     hipLaunchKernelGGL(k_add_f32_to_f64, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*u), view_of<float>(*e),
                        bx, g.zchunks);
+    return launch_status();
+}
+
+int mgp_fill_modes_f64(const mgp_array* f, int64_t nmodes, const int32_t* abc, const double* amp, double h,
+                       void* s) {
+    if (!valid_array(f) || nmodes < 0 || nmodes > MGP_MAX_MODES || (nmodes && (!abc || !amp)) || !(h > 0.0))
+        return MGP_ERR_ARG;
+    Modes m;
+    m.n = (int32_t)nmodes;
+    for (int64_t q = 0; q < nmodes; q++) {
+        for (int d = 0; d < 3; d++) m.abc[3 * q + d] = abc[3 * q + d];
+        m.amp[q] = amp[q];
+    }
+    const Box bx = shape_box(f);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL(k_fill_modes<double>, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*f), bx, g.zchunks,
+                       m, h);
     return launch_status();
 }
 

@@ -123,6 +123,7 @@ int mgp_tuning_set(const char* key, int64_t value) {
     else if (!strcmp(key, "sweep.waves") && (value == 4 || value == 8 || value == 16)) mgp::g_sweep_waves = value;
     else if (!strcmp(key, "sweep.kernel") && (value == 1 || value == 2)) mgp::g_sweep_kernel = value;
     else if (!strcmp(key, "sweep.rows") && (value == 8 || value == 12 || value == 16)) mgp::g_sweep_rows = value;
+    else if (!strcmp(key, "sweep.depth") && (value == 1 || value == 2)) mgp::g_sweep_depth = value;
     else return MGP_ERR_ARG;
     return MGP_OK;
 }
@@ -134,6 +135,7 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "sweep.waves")) *value = mgp::g_sweep_waves;
     else if (!strcmp(key, "sweep.kernel")) *value = mgp::g_sweep_kernel;
     else if (!strcmp(key, "sweep.rows")) *value = mgp::g_sweep_rows;
+    else if (!strcmp(key, "sweep.depth")) *value = mgp::g_sweep_depth;
     else return MGP_ERR_ARG;
     return MGP_OK;
 }

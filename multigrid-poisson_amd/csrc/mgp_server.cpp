@@ -1,0 +1,422 @@
+// Serving loop (mgp_server_*): Poisson solves of mixed shapes and boundary
+// types accepted as they arrive and run concurrently on one GPU.  No
+// counterpart in the reference, which runs one solve per process.
+//
+// A worker thread owns the device work.  Each admitted request gets a slot: a
+// RefinementSolver (FP32 F-cycles inside FP64 iterative refinement) with its
+// own stream.  Slots stay resident after their request ends and are reused by
+// later requests of the same shape and boundary types; an HBM budget bounds
+// active plus resident slots, and idle slots are evicted least recently used
+// first when a new shape needs room.  The worker admits pending requests in
+// arrival order while a stream slot and memory are free, enqueues one
+// refinement iteration per active solve at a time, and polls each solve's
+// event; so iterations of different requests overlap on the device and a
+// request that arrives mid-way joins at the next poll.  A request's arithmetic
+// does not depend on what else runs (each solve has its own arrays and
+// stream, every kernel is deterministic), so its result is the same bits
+// alone or in a mix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <list>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <thread>
+#include <tuple>
+#include <vector>
+
+#include "libpoisson.h"
+#include "mgp.h"
+
+namespace mgp {
+void set_last_error(const char* msg);
+}
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+using Key = std::tuple<int64_t, int64_t, int64_t, uint32_t>;  // dims, face-type bits
+
+double seconds(Clock::time_point a, Clock::time_point b) { return std::chrono::duration<double>(b - a).count(); }
+
+struct Job {
+    int64_t id = 0;
+    mgp_request req{};
+    Clock::time_point t_submit, t_admit, t_done;
+    mgp_result res{};
+    bool done = false;
+};
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw std::runtime_error(std::string(what) + ": " + hipGetErrorString(e));
+}
+
+// levels down to a 3-point coarsest axis with exact 2:1 nesting
+int64_t max_levels(const int64_t d[3]) {
+    int64_t L = 1;
+    while (true) {
+        bool ok = true;
+        for (int a = 0; a < 3; a++) {
+            const int64_t s = (d[a] - 1) >> (L - 1);
+            if (s % 2 != 0 || s / 2 < 2) ok = false;
+        }
+        if (!ok) return L;
+        L++;
+    }
+}
+
+struct SlotBase {
+    Key key;
+    size_t bytes = 0;
+    Job* job = nullptr;
+    hipEvent_t ev = nullptr;
+    std::vector<double> staging;  // host copy of the padded solution
+    virtual ~SlotBase() {
+        if (ev) (void)hipEventDestroy(ev);
+    }
+    virtual void begin(const mgp_request& r) = 0;  // f, u = 0, ||f||, initial residual (waits)
+    virtual void enqueue() = 0;                     // one iteration + its norm, then the event
+    virtual bool step_done() = 0;                   // stop rule after the event
+    virtual void finish(double* out) = 0;           // solution to the host (waits)
+    virtual void stats(int64_t* iters, double* rel) = 0;
+};
+
+template <template <class> class S>
+struct Slot : SlotBase {
+    Poisson::Settings st;
+    Poisson::BoundaryCondition bc;
+    std::unique_ptr<Poisson::RefinementSolver<Poisson::FullWeighting, Poisson::TrilinearInterpolation, S>> solver;
+    int64_t nsmooth;
+    double omega;
+
+    Slot(const mgp_request& r, const mgp_server_config& c) {
+        for (int d = 0; d < 3; d++) {
+            st.dims[d] = r.dims[d];
+            st.origin[d] = 0.0;
+        }
+        st.levels = max_levels(r.dims);
+        st.h = 1.0 / (double)(r.dims[0] - 1);
+        st.lengthx = 1.0;
+        st.maxiter = c.maxiter;
+        st.miniter = 1;
+        st.tolerance = c.tolerance;
+        st.coarse_sweeps = c.coarse_sweeps;
+        st.mode = Poisson::CONSISTENT;
+        st.print_cycle_time = false;
+        auto t = [&](int q) { return r.bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
+        bc = Poisson::BoundaryCondition{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
+        solver.reset(new Poisson::RefinementSolver<Poisson::FullWeighting, Poisson::TrilinearInterpolation, S>(st, bc));
+        solver->init_zero();
+        solver->to_device();
+        nsmooth = c.nsmooth;
+        omega = c.omega;
+        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "server event");
+    }
+    void begin(const mgp_request& r) override {
+        Poisson::Domain<double>& d = solver->fp64().get_domain(0, 0);
+        const mgp_array f = d.f->desc();
+        Poisson::mgpx::check(mgp_fill_modes_f64(&f, r.nmodes, r.abc, r.amp, st.h, solver->get_stream()), "rhs");
+        solver->reset_solution();  // u = 0 with the (zero) boundary data; waits
+        solver->start("Fcycle");
+    }
+    void enqueue() override {
+        solver->enqueue_iteration(nsmooth, omega);
+        hip_check(hipEventRecord(ev, (hipStream_t)solver->get_stream()), "server event");
+    }
+    bool step_done() override { return solver->iteration_done(); }
+    void finish(double* out) override {
+        const mgp_array u = solver->fp64().get_domain(0, 0).u->desc();
+        const int64_t p0 = u.shape[0] + u.halo[MGP_EAST] + u.halo[MGP_WEST];
+        const int64_t n = p0 * u.stride[0];
+        staging.resize(n);
+        hip_check(hipMemcpyAsync(staging.data(), u.data, n * sizeof(double), hipMemcpyDeviceToHost,
+                                 (hipStream_t)solver->get_stream()),
+                  "server result");
+        hip_check(hipStreamSynchronize((hipStream_t)solver->get_stream()), "server result");
+        if (!out) return;
+        const int64_t base = u.halo[MGP_WEST] * u.stride[0] + u.halo[MGP_SOUTH] * u.stride[1] + u.halo[MGP_BOTTOM];
+        for (int64_t i = 0; i < u.shape[0]; i++)
+            for (int64_t j = 0; j < u.shape[1]; j++)
+                std::memcpy(out + (i * u.shape[1] + j) * u.shape[2], staging.data() + base + i * u.stride[0] + j * u.stride[1],
+                            u.shape[2] * sizeof(double));
+    }
+    void stats(int64_t* iters, double* rel) override {
+        *iters = (int64_t)solver->solve_iterations();
+        *rel = solver->relative_residual();
+    }
+};
+
+class Server {
+public:
+    explicit Server(const mgp_server_config& c) : cfg(c) {
+        if (c.hbm_budget <= 0 || c.max_active < 1 || c.nsmooth < 1 || c.maxiter < 1 || !(c.tolerance >= 0.0) ||
+            (c.relaxation != MGP_GAUSS_SEIDEL && c.relaxation != MGP_JACOBI))
+            throw std::invalid_argument("invalid server config");
+        worker = std::thread([this] { run(); });
+    }
+    ~Server() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stopping = true;
+        }
+        cv_work.notify_all();
+        worker.join();
+    }
+
+    int64_t submit(const mgp_request& r) {
+        for (int d = 0; d < 3; d++)
+            if (r.dims[d] < 3) throw std::invalid_argument("dims must be >= 3");
+        bool any_dirichlet = false;
+        for (int q = 0; q < 6; q++) {
+            if (r.bc[q] != MGP_DIRICHLET && r.bc[q] != MGP_NEUMANN) throw std::invalid_argument("bad face type");
+            any_dirichlet |= r.bc[q] == MGP_DIRICHLET;
+        }
+        // all-Neumann problems are singular (solution up to a constant)
+        if (!any_dirichlet) throw std::invalid_argument("all-Neumann requests are not accepted");
+        if (r.nmodes < 0 || r.nmodes > MGP_MAX_MODES) throw std::invalid_argument("nmodes out of range");
+        int64_t tmp[3] = {r.dims[0], r.dims[1], r.dims[2]};
+        if (max_levels(tmp) < 2) throw std::invalid_argument("shape does not coarsen (need 2^k m + 1 axes)");
+        auto j = std::make_unique<Job>();
+        j->req = r;
+        j->t_submit = Clock::now();
+        std::lock_guard<std::mutex> lk(mu);
+        j->id = next_id++;
+        const int64_t id = j->id;
+        pending.push_back(j.get());
+        jobs[id] = std::move(j);
+        cv_work.notify_all();
+        return id;
+    }
+
+    mgp_result wait(int64_t id) {
+        std::unique_lock<std::mutex> lk(mu);
+        auto it = jobs.find(id);
+        if (it == jobs.end()) throw std::invalid_argument("unknown request id");
+        Job* j = it->second.get();
+        cv_done.wait(lk, [&] { return j->done; });
+        const mgp_result r = j->res;
+        jobs.erase(it);
+        return r;
+    }
+
+    void stats(int64_t* completed_out, int64_t* peak_out, int64_t* resident_out) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (completed_out) *completed_out = completed;
+        if (peak_out) *peak_out = peak_active;
+        if (resident_out) *resident_out = (int64_t)bytes_used;
+    }
+
+private:
+    mgp_server_config cfg;
+    std::thread worker;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    bool stopping = false;
+    int64_t next_id = 1;
+    std::deque<Job*> pending;
+    std::map<int64_t, std::unique_ptr<Job>> jobs;
+    // worker-owned
+    std::vector<std::unique_ptr<SlotBase>> active;
+    std::list<std::unique_ptr<SlotBase>> idle;  // least recently used first
+    size_t bytes_used = 0;
+    int64_t completed = 0, peak_active = 0;
+
+    static Key key_of(const mgp_request& r) {
+        uint32_t bits = 0;
+        for (int q = 0; q < 6; q++) bits |= (uint32_t)(r.bc[q] == MGP_NEUMANN) << q;
+        return Key{r.dims[0], r.dims[1], r.dims[2], bits};
+    }
+
+    std::unique_ptr<SlotBase> make_slot(const mgp_request& r) {
+        size_t free0 = 0, total = 0, free1 = 0;
+        hip_check(hipMemGetInfo(&free0, &total), "server");
+        std::unique_ptr<SlotBase> s;
+        if (cfg.relaxation == MGP_GAUSS_SEIDEL) s.reset(new Slot<Poisson::GaussSeidel>(r, cfg));
+        else s.reset(new Slot<Poisson::Jacobi>(r, cfg));
+        hip_check(hipMemGetInfo(&free1, &total), "server");
+        s->key = key_of(r);
+        s->bytes = free0 > free1 ? free0 - free1 : 0;
+        return s;
+    }
+
+    // a resident slot for the request, or nullptr when memory does not allow one now
+    std::unique_ptr<SlotBase> acquire(const mgp_request& r) {
+        const Key k = key_of(r);
+        for (auto it = idle.begin(); it != idle.end(); ++it)
+            if ((*it)->key == k) {
+                std::unique_ptr<SlotBase> s = std::move(*it);
+                idle.erase(it);
+                return s;
+            }
+        // estimate: level 0 in FP64 (u, u_alt, f, r) plus the FP32 hierarchy
+        // (u, u_alt, f, r; 8/7 of level 0) plus padding
+        const double pts = (double)(r.dims[0] + 2) * (double)(r.dims[1] + 2) * (double)(r.dims[2] + 2);
+        const size_t need = (size_t)(pts * (4 * 8 + 4 * 4 * 8.0 / 7.0) * 1.05) + (8u << 20);
+        while (bytes_used + need > (size_t)cfg.hbm_budget && !idle.empty()) {
+            bytes_used -= idle.front()->bytes;
+            idle.pop_front();  // frees its device arrays
+        }
+        if (bytes_used + need > (size_t)cfg.hbm_budget && !active.empty()) return nullptr;  // wait for room
+        std::unique_ptr<SlotBase> s = make_slot(r);
+        bytes_used += s->bytes;
+        return s;
+    }
+
+    void complete(SlotBase& s, int status) {
+        Job* j = s.job;
+        if (status == MGP_OK) {
+            try {
+                s.finish(j->req.u_out);
+                s.stats(&j->res.iterations, &j->res.rel_res);
+            } catch (const std::exception& e) {
+                mgp::set_last_error(e.what());
+                status = MGP_ERR_STATE;
+            }
+        }
+        s.job = nullptr;
+        complete_job(j, status);
+    }
+    void complete_job(Job* j, int status) {
+        j->t_done = Clock::now();
+        std::lock_guard<std::mutex> lk(mu);
+        j->res.status = status;
+        j->res.latency_s = seconds(j->t_submit, j->t_done);
+        j->res.queue_s = seconds(j->t_submit, j->t_admit);
+        j->res.run_s = seconds(j->t_admit, j->t_done);
+        j->done = true;
+        completed++;
+        cv_done.notify_all();
+    }
+
+    void run() {
+        while (true) {
+            // admission, in arrival order
+            while ((int)active.size() < cfg.max_active) {
+                Job* j = nullptr;
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    if (pending.empty()) break;
+                    j = pending.front();
+                }
+                std::unique_ptr<SlotBase> s;
+                try {
+                    s = acquire(j->req);
+                } catch (const std::exception& e) {
+                    // e.g. out of device memory with nothing else running
+                    mgp::set_last_error(e.what());
+                    {
+                        std::lock_guard<std::mutex> lk(mu);
+                        pending.pop_front();
+                    }
+                    j->t_admit = Clock::now();
+                    complete_job(j, MGP_ERR_STATE);
+                    continue;
+                }
+                if (!s) break;  // no memory until an active solve ends
+                {
+                    std::lock_guard<std::mutex> lk(mu);
+                    pending.pop_front();
+                }
+                j->t_admit = Clock::now();
+                s->job = j;
+                try {
+                    s->begin(j->req);
+                    s->enqueue();
+                } catch (const std::exception& e) {
+                    mgp::set_last_error(e.what());
+                    complete(*s, MGP_ERR_STATE);
+                    idle.push_back(std::move(s));
+                    continue;
+                }
+                active.push_back(std::move(s));
+                std::lock_guard<std::mutex> lk(mu);
+                peak_active = std::max<int64_t>(peak_active, (int64_t)active.size());
+            }
+            // progress of the active solves
+            bool progressed = false;
+            for (size_t a = 0; a < active.size();) {
+                SlotBase& s = *active[a];
+                const hipError_t q = hipEventQuery(s.ev);
+                if (q == hipErrorNotReady) {
+                    a++;
+                    continue;
+                }
+                progressed = true;
+                bool end = true;
+                int status = MGP_OK;
+                try {
+                    hip_check(q, "server iteration");
+                    end = s.step_done();
+                    if (!end) s.enqueue();
+                } catch (const std::exception& e) {
+                    mgp::set_last_error(e.what());
+                    status = MGP_ERR_STATE;
+                    end = true;
+                }
+                if (!end) {
+                    a++;
+                    continue;
+                }
+                complete(s, status);
+                idle.push_back(std::move(active[a]));
+                active.erase(active.begin() + a);
+            }
+            std::unique_lock<std::mutex> lk(mu);
+            if (stopping && pending.empty() && active.empty()) break;
+            if (active.empty() && pending.empty()) {
+                cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
+            } else if (!progressed) {
+                lk.unlock();
+                std::this_thread::sleep_for(std::chrono::microseconds(20));
+            }
+        }
+        active.clear();
+        idle.clear();
+    }
+};
+
+template <class F>
+int server_guarded(F&& f) {
+    try {
+        f();
+        return MGP_OK;
+    } catch (const std::invalid_argument& e) {
+        mgp::set_last_error(e.what());
+        return MGP_ERR_ARG;
+    } catch (const std::exception& e) {
+        mgp::set_last_error(e.what());
+        return MGP_ERR_STATE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int mgp_server_create(const mgp_server_config* c, void** out) {
+    if (!c || !out) return MGP_ERR_ARG;
+    return server_guarded([&] { *out = new Server(*c); });
+}
+int mgp_server_destroy(void* s) {
+    return server_guarded([&] { delete static_cast<Server*>(s); });
+}
+int mgp_server_submit(void* s, const mgp_request* r, int64_t* id) {
+    if (!s || !r || !id) return MGP_ERR_ARG;
+    return server_guarded([&] { *id = static_cast<Server*>(s)->submit(*r); });
+}
+int mgp_server_wait(void* s, int64_t id, mgp_result* out) {
+    if (!s || !out) return MGP_ERR_ARG;
+    return server_guarded([&] { *out = static_cast<Server*>(s)->wait(id); });
+}
+int mgp_server_stats(void* s, int64_t* completed, int64_t* peak_active, int64_t* bytes_resident) {
+    if (!s) return MGP_ERR_ARG;
+    return server_guarded([&] { static_cast<Server*>(s)->stats(completed, peak_active, bytes_resident); });
+}
+
+}  // extern "C"

@@ -163,6 +163,10 @@ SolverBase* S(void* h) { return static_cast<SolverBase*>(h); }
 
 }  // namespace
 
+namespace mgp {
+void set_last_error(const char* msg) { g_last_error = msg; }
+}  // namespace mgp
+
 extern "C" {
 
 int mgp_solver_create(const mgp_solver_config* c, void** out) {

@@ -18,4 +18,5 @@ extern int64_t g_sweep_min_chunk;
 extern int64_t g_sweep_waves;
 extern int64_t g_sweep_kernel;
 extern int64_t g_sweep_rows;
+extern int64_t g_sweep_depth;
 }  // namespace mgp

@@ -392,7 +392,8 @@ __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32
 // EDGE: the tile has rows outside the box or in-plane Neumann ghost positions
 // (boundary tiles only); the other tiles run a copy of the loop without the
 // row masks and the ghost code
-template <class T, int MODE, int R, bool EDGE>
+// D: planes of loads in flight ahead of the red update (1 or 2)
+template <class T, int MODE, int R, int D, bool EDGE>
 __device__ __forceinline__ void wave_march(const MarchArgs<T>& A, int tl) {
     constexpr int NR = R + 2;
     const int tz = tl % A.ztiles;
@@ -500,19 +501,25 @@ __device__ __forceinline__ void wave_march(const MarchArgs<T>& A, int tl) {
         }
     };
 
-    T Pm2[NR], Pm1[NR], P0[NR], Pp1[NR], Pp2[NR], Fm1[NR], F0[NR], Fp1[NR];
+    // x window: planes i-2 .. i+1 (+ i+2 when D == 2) and the plane being
+    // loaded; f of planes i-1, i (+ i+1) and the f plane being loaded
+    T Pm2[NR], Pm1[NR], P0[NR], Pp1[NR], Pp2[NR], PL[NR], Fm1[NR], F0[NR], Fp1[NR], FL[NR];
     const int64_t i0 = xa - 1;
     load_plane(i0 - 1, Pm1);
     load_plane(i0, P0);
     load_plane(i0 + 1, Pp1);
     load_f(i0, F0);
+    if (D == 2) {
+        load_plane(i0 + 2, Pp2);
+        load_f(i0 + 1, Fp1);
+    }
 #pragma unroll
     for (int t = 0; t < NR; t++) Pm2[t] = Fm1[t] = (T)0;
 
     for (int64_t i = i0; i <= xb; i++) {
-        // next planes in flight during this step
-        load_plane(i + 2, Pp2);
-        load_f(i + 1, Fp1);
+        // next planes in flight during this step (and the next, when D == 2)
+        load_plane(i + 1 + D, PL);
+        load_f(i + D, FL);
         const bool plane_in = i >= 0 && i < n0;  // uniform
         const bool x_in = i >= A.bx0 && i < A.bx1;
         const int par = (int)(i & 1);
@@ -565,14 +572,21 @@ __device__ __forceinline__ void wave_march(const MarchArgs<T>& A, int tl) {
             Pm2[t] = Pm1[t];
             Pm1[t] = P0[t];
             P0[t] = Pp1[t];
-            Pp1[t] = Pp2[t];
             Fm1[t] = F0[t];
-            F0[t] = Fp1[t];
+            if (D == 2) {
+                Pp1[t] = Pp2[t];
+                Pp2[t] = PL[t];
+                F0[t] = Fp1[t];
+                Fp1[t] = FL[t];
+            } else {
+                Pp1[t] = PL[t];
+                F0[t] = FL[t];
+            }
         }
     }
 }
 
-template <class T, int MODE, int R>
+template <class T, int MODE, int R, int D>
 __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
@@ -583,8 +597,8 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
     const bool edge = y0 - 1 < A.by0 || y0 + R - 2 >= A.by1 || (y0 - 2 < 0 && A.g[MGP_SOUTH]) ||
                       (y0 + R - 1 >= A.ui.n1 && A.g[MGP_NORTH]) || (z0 - 2 < 0 && A.g[MGP_BOTTOM]) ||
                       (z0 + WZ + 1 >= A.ui.n2 && A.g[MGP_TOP]);
-    if (edge) wave_march<T, MODE, R, true>(A, tl);
-    else wave_march<T, MODE, R, false>(A, tl);
+    if (edge) wave_march<T, MODE, R, D, true>(A, tl);
+    else wave_march<T, MODE, R, D, false>(A, tl);
 }
 
 }  // namespace
@@ -628,12 +642,13 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
         A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
         const dim3 g(A.ntiles_pad);
-#define MGP_WAVE_LAUNCH(RR)                                                                          \
-    if (ref) hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_REFERENCE, RR>), g, dim3(64), 0, st, A); \
-    else hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_CONSISTENT, RR>), g, dim3(64), 0, st, A);
-        if (rows == 8) { MGP_WAVE_LAUNCH(8) }
-        else if (rows == 12) { MGP_WAVE_LAUNCH(12) }
-        else { MGP_WAVE_LAUNCH(16) }
+#define MGP_WAVE_LAUNCH(RR, DD)                                                                          \
+    if (ref) hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_REFERENCE, RR, DD>), g, dim3(64), 0, st, A); \
+    else hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_CONSISTENT, RR, DD>), g, dim3(64), 0, st, A);
+        const bool d2 = g_sweep_depth == 2;
+        if (rows == 8) { if (d2) { MGP_WAVE_LAUNCH(8, 2) } else { MGP_WAVE_LAUNCH(8, 1) } }
+        else if (rows == 12) { if (d2) { MGP_WAVE_LAUNCH(12, 2) } else { MGP_WAVE_LAUNCH(12, 1) } }
+        else { if (d2) { MGP_WAVE_LAUNCH(16, 2) } else { MGP_WAVE_LAUNCH(16, 1) } }
 #undef MGP_WAVE_LAUNCH
         const hipError_t e = hipGetLastError();
         return e == hipSuccess ? MGP_OK : (int)e;
@@ -663,6 +678,7 @@ int64_t g_sweep_min_chunk = 48;     // shortest x march per workgroup
 int64_t g_sweep_waves = 8;          // waves per workgroup: 4, 8 or 16
 int64_t g_sweep_kernel = 1;         // 1: workgroup tiles with LDS, 2: wave-private tiles
 int64_t g_sweep_rows = 16;          // register rows per wave tile: 8, 12 or 16
+int64_t g_sweep_depth = 1;          // planes of loads in flight per wave: 1 or 2
 
 // x-chunk length: enough workgroups to keep every CU busy with several
 // marches, but chunks long enough that the two warm-up planes per chunk stay

@@ -44,6 +44,33 @@ class SolverConfig(ctypes.Structure):
                 ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64)]
 
 
+class BatchConfig(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_int64), ("levels", ctypes.c_int64), ("relaxation", ctypes.c_int32),
+                ("flags", ctypes.c_int32), ("nsmooth", ctypes.c_int64), ("omega", ctypes.c_double),
+                ("coarse_sweeps", ctypes.c_int64), ("maxiter", ctypes.c_int64), ("miniter", ctypes.c_int64),
+                ("tolerance", ctypes.c_double)]
+
+
+BATCH_NO_GRAPHS, BATCH_RETUNE = 1, 2
+MAX_MODES = 16
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_int64), ("max_active", ctypes.c_int32), ("relaxation", ctypes.c_int32),
+                ("nsmooth", ctypes.c_int64), ("omega", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64),
+                ("maxiter", ctypes.c_int64), ("tolerance", ctypes.c_double)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("dims", ctypes.c_int64 * 3), ("bc", ctypes.c_int32 * 6), ("nmodes", ctypes.c_int32),
+                ("abc", ctypes.c_int32 * (3 * MAX_MODES)), ("amp", ctypes.c_double * MAX_MODES),
+                ("u_out", ctypes.c_void_p)]
+
+
+class Result(ctypes.Structure):
+    _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int64), ("rel_res", ctypes.c_double),
+                ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double), ("run_s", ctypes.c_double)]
+
 _lib = None
 P = ctypes.POINTER
 vp, i32, i64, d, f = ctypes.c_void_p, ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_float
@@ -68,6 +95,13 @@ _SIGS = {
     "mgp_solver_cycle": [vp, ctypes.c_char_p, i64, d], "mgp_solver_relres": [vp, P(d)], "mgp_solver_sync": [vp], "mgp_solver_reset": [vp],
     "mgp_solver_stats": [vp, P(d), P(d), P(i64)], "mgp_solver_array": [vp, i32, i32, P(vp), P(Array)],
     "mgp_solver_face": [vp, i32, i32, P(vp), P(Array), P(ctypes.c_int)], "mgp_solver_save": [vp, ctypes.c_char_p],
+    "mgp_fill_modes_f64": [P(Array), i64, P(ctypes.c_int32), P(d), d, vp],
+    "mgp_batch_create": [P(BatchConfig), P(i64), P(vp)], "mgp_batch_destroy": [vp],
+    "mgp_batch_set_rhs": [vp, i64, P(d)], "mgp_batch_solve": [vp, P(d)], "mgp_batch_stats": [vp, i64, P(d), P(i64)],
+    "mgp_batch_solution": [vp, i64, P(d)], "mgp_batch_tile": [vp, i64, i64, P(i64)],
+    "mgp_server_create": [P(ServerConfig), P(vp)], "mgp_server_destroy": [vp],
+    "mgp_server_submit": [vp, P(Request), P(i64)], "mgp_server_wait": [vp, i64, P(Result)],
+    "mgp_server_stats": [vp, P(i64), P(i64), P(i64)],
 }
 for sfx in ("f64", "f32"):
     _SIGS.update({
@@ -329,3 +363,117 @@ class Solver:
 
     def save(self, path):
         check(lib().mgp_solver_save(self.h, path.encode()), "save")
+
+
+class Batch:
+    """Batched FP32-cycle / FP64-refinement solves of zero-Dirichlet problems
+    (mgp_batch_*): dims is a list of (nx, ny, nz)."""
+
+    def __init__(self, dims, relaxation=JACOBI, nsmooth=3, omega=-1.0, levels=0, coarse_sweeps=0, maxiter=50,
+                 miniter=1, tolerance=1e-10, flags=0):
+        require_gpu()
+        self.dims = [tuple(int(v) for v in d) for d in dims]
+        c = BatchConfig()
+        c.count, c.levels, c.relaxation, c.flags = len(self.dims), int(levels), int(relaxation), int(flags)
+        c.nsmooth, c.omega, c.coarse_sweeps = int(nsmooth), float(omega), int(coarse_sweeps)
+        c.maxiter, c.miniter, c.tolerance = int(maxiter), int(miniter), float(tolerance)
+        flat = (ctypes.c_int64 * (3 * len(self.dims)))(*[v for d in self.dims for v in d])
+        h = vp()
+        check(lib().mgp_batch_create(ctypes.byref(c), flat, ctypes.byref(h)), "mgp_batch_create")
+        self.h = h.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgp_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_rhs(self, p, f):
+        f = np.ascontiguousarray(f, dtype=np.float64)
+        assert f.size == int(np.prod(self.dims[p]))
+        check(lib().mgp_batch_set_rhs(self.h, int(p), f.ctypes.data_as(P(d))), "mgp_batch_set_rhs")
+
+    def solve(self):
+        t = ctypes.c_double()
+        check(lib().mgp_batch_solve(self.h, ctypes.byref(t)), "mgp_batch_solve")
+        return t.value
+
+    def stats(self, p):
+        rr, it = ctypes.c_double(), ctypes.c_int64()
+        check(lib().mgp_batch_stats(self.h, int(p), ctypes.byref(rr), ctypes.byref(it)), "mgp_batch_stats")
+        return it.value, rr.value
+
+    def solution(self, p):
+        out = np.empty(self.dims[p], dtype=np.float64)
+        check(lib().mgp_batch_solution(self.h, int(p), out.ctypes.data_as(P(d))), "mgp_batch_solution")
+        return out
+
+    def tile(self, p, level):
+        out = (ctypes.c_int64 * 2)()
+        check(lib().mgp_batch_tile(self.h, int(p), int(level), out), "mgp_batch_tile")
+        return out[0], out[1]
+
+
+def make_request(dims, bc, modes, u_out=None):
+    """mgp_request for a zero-boundary-data solve with f = sum of Fourier modes;
+    modes = [(a, b, c, amp), ...]; bc = 6 characters 'D'/'N' (N S E W T B)."""
+    r = Request()
+    r.dims[:] = [int(v) for v in dims]
+    r.bc[:] = [NEUMANN if ch == "N" else DIRICHLET for ch in bc]
+    r.nmodes = len(modes)
+    for m, (a, b, c, amp) in enumerate(modes):
+        r.abc[3 * m], r.abc[3 * m + 1], r.abc[3 * m + 2] = int(a), int(b), int(c)
+        r.amp[m] = float(amp)
+    r.u_out = u_out.ctypes.data if u_out is not None else None
+    return r
+
+
+class Server:
+    """The serving loop (mgp_server_*)."""
+
+    def __init__(self, hbm_budget=64 << 30, max_active=4, relaxation=GAUSS_SEIDEL, nsmooth=2, omega=1.0,
+                 coarse_sweeps=32, maxiter=100, tolerance=1e-8):
+        require_gpu()
+        c = ServerConfig()
+        c.hbm_budget, c.max_active, c.relaxation = int(hbm_budget), int(max_active), int(relaxation)
+        c.nsmooth, c.omega, c.coarse_sweeps = int(nsmooth), float(omega), int(coarse_sweeps)
+        c.maxiter, c.tolerance = int(maxiter), float(tolerance)
+        h = vp()
+        check(lib().mgp_server_create(ctypes.byref(c), ctypes.byref(h)), "mgp_server_create")
+        self.h = h.value
+        self._keep = {}
+
+    def submit(self, dims, bc, modes, want_solution=True):
+        """Queue a request; returns its id (the solution array is kept until wait)."""
+        out = np.empty(tuple(int(v) for v in dims), dtype=np.float64) if want_solution else None
+        r = make_request(dims, bc, modes, out)
+        i = ctypes.c_int64()
+        check(lib().mgp_server_submit(self.h, ctypes.byref(r), ctypes.byref(i)), "mgp_server_submit")
+        self._keep[i.value] = out
+        return i.value
+
+    def wait(self, rid):
+        res = Result()
+        check(lib().mgp_server_wait(self.h, int(rid), ctypes.byref(res)), "mgp_server_wait")
+        return res, self._keep.pop(rid)
+
+    def stats(self):
+        a, b, c = ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+        check(lib().mgp_server_stats(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)), "mgp_server_stats")
+        return a.value, b.value, c.value
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().mgp_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

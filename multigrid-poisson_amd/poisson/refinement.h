@@ -82,43 +82,71 @@ public:
         using clock = std::chrono::steady_clock;
         const auto t0 = clock::now();
         auto since = [&]() { return std::chrono::duration<double>(clock::now() - t0).count(); };
-        const bool v = solver == "vcycle" || solver == "Vcycle";
-        const bool f = solver == "fcycle" || solver == "Fcycle";
-        if (!(v || f)) throw std::invalid_argument("RefinementSolver: solver must be \"Vcycle\" or \"Fcycle\"");
         (void)ofile;
-        void* st = outer->get_stream();
-        mgpx::StreamScope scope(st);
-        Domain<double>& d64 = outer->get_domain(0, 0);
-        Domain<float>& d32 = inner->get_domain(0, 0);
-        const double fnorm = d64.f->infinity_norm();
-        double norm = level0_norm(d64, st) / fnorm;
-        rel_res = norm;
+        start(solver);
         if (is_verbose) std::cout << std::setw(4) << 0 << ": Initial residual: " << std::setw(8) << rel_res << std::endl;
-        for (iter = 0; iter < settings.maxiter; iter++) {
-            // correction equation: f32 = r, e = 0, one FP32 cycle
-            const mgp_array r64 = d64.r->desc(), f32 = d32.f->desc(), e32 = d32.u->desc(), u64 = d64.u->desc();
-            mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
-            d32.u->init_zero();
-            inner->cycle(f ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
-            // u += e, then the FP64 residual and its norm
-            const mgp_array e32b = d32.u->desc();
-            (void)e32;
-            mgpx::check(mgp_add_f32_to_f64(&u64, &e32b, st), "refinement: update");
-            norm = level0_norm(d64, st) / fnorm;
-            if (is_verbose) std::cout << std::setw(4) << iter + 1 << ": Relative residual: " << std::setw(8) << norm << std::endl;
-            if (norm <= settings.tolerance) {
-                rel_res = norm;
-                break;
+        if (settings.maxiter > 0) {
+            while (true) {
+                enqueue_iteration(nsmooth, omega);
+                mgpx::check(mgp_stream_sync(outer->get_stream()), "refinement");
+                if (is_verbose)
+                    std::cout << std::setw(4) << iter + 1 << ": Relative residual: " << std::setw(8)
+                              << host_norm[0] / fnorm << std::endl;
+                if (iteration_done(since() > maxtime)) break;
             }
-            if (iter >= settings.miniter && (norm > 2.0 * rel_res || since() > maxtime)) {
-                rel_res = norm;
-                break;
-            }
-            rel_res = norm;
         }
         wtime = since();
     }
 
+    // ---- the same iteration split for callers that drive many solves
+    // without blocking (one stream each): start() (waits: ||f|| and the
+    // initial residual), then per iteration enqueue_iteration() and, once
+    // the stream has passed it (event after it), iteration_done(), which
+    // applies the stop rule of solve() and returns true when the solve ends.
+    void start(const std::string& solver) {
+        const bool v = solver == "vcycle" || solver == "Vcycle";
+        const bool f = solver == "fcycle" || solver == "Fcycle";
+        if (!(v || f)) throw std::invalid_argument("RefinementSolver: solver must be \"Vcycle\" or \"Fcycle\"");
+        fcycle = f;
+        void* st = outer->get_stream();
+        mgpx::StreamScope scope(st);
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        fnorm = d64.f->infinity_norm();
+        enqueue_norm(d64, st);
+        mgpx::check(mgp_stream_sync(st), "refinement: norm");
+        rel_res = host_norm[0] / fnorm;
+        iter = 0;
+    }
+    // f32 = r, e = 0, one FP32 cycle, u += e, then the FP64 residual and its
+    // norm (copied to host memory the stream writes when it gets there)
+    void enqueue_iteration(int_t nsmooth, double omega) {
+        void* st = outer->get_stream();
+        mgpx::StreamScope scope(st);
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        Domain<float>& d32 = inner->get_domain(0, 0);
+        const mgp_array r64 = d64.r->desc(), f32 = d32.f->desc(), u64 = d64.u->desc();
+        mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
+        d32.u->init_zero();
+        inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
+        const mgp_array e32 = d32.u->desc();
+        mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
+        enqueue_norm(d64, st);
+    }
+    // stop rule of solve(); `timed_out` stands for the wall-time limit
+    bool iteration_done(bool timed_out = false) {
+        const double norm = host_norm[0] / fnorm;
+        if (norm <= settings.tolerance) {
+            rel_res = norm;
+            return true;
+        }
+        if (iter >= settings.miniter && (norm > 2.0 * rel_res || timed_out)) {
+            rel_res = norm;
+            return true;
+        }
+        rel_res = norm;
+        iter++;
+        return iter >= settings.maxiter;
+    }
     double relative_residual() { return rel_res; }
     double solve_time() { return wtime; }
     double solve_iterations() { return (double)iter; }
@@ -132,12 +160,10 @@ public:
     void* get_stream() const { return outer->get_stream(); }
 
 private:
-    double level0_norm(Domain<double>& d, void* st) {
+    void enqueue_norm(Domain<double>& d, void* st) {
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
         residual_norm<double>(d, dev_norm);
         mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
-        mgpx::check(mgp_stream_sync(st), "refinement: norm");
-        return host_norm[0];
     }
 
     Settings settings, outer_settings, inner_settings;
@@ -147,8 +173,9 @@ private:
     double* dev_norm = nullptr;
     double* host_norm = nullptr;
     bool is_verbose = false;
+    bool fcycle = true;
     uint_t iter = 0;
-    double rel_res = 0.0, wtime = 0.0;
+    double rel_res = 0.0, wtime = 0.0, fnorm = 1.0;
 };
 }  // namespace Poisson
 #endif

@@ -300,26 +300,28 @@ SWEEP_SHAPES = [((40, 33, 130), "NNNNDD"), ((9, 17, 65), "DDDDDD"), ((33, 30, 64
                 ((9, 29, 125), "NNNNNN")]
 
 
-SWEEP_KERNELS = [(1, 16), (2, 8), (2, 12), (2, 16)]  # (sweep.kernel, sweep.rows)
+# (sweep.kernel, sweep.rows, sweep.depth)
+SWEEP_KERNELS = [(1, 16, 1), (2, 8, 1), (2, 12, 1), (2, 16, 1), (2, 8, 2), (2, 16, 2)]
 
 
 @pytest.fixture
 def sweep_kernel(request):
-    k, rows = request.param
     L = mgp.lib()
+    keys = ("sweep.kernel", "sweep.rows", "sweep.depth")
     old = {}
-    for key in ("sweep.kernel", "sweep.rows"):
+    for key in keys:
         v = ctypes.c_int64()
         mgp.check(L.mgp_tuning_get(key.encode(), ctypes.byref(v)))
         old[key] = v.value
-    mgp.check(L.mgp_tuning_set(b"sweep.kernel", k))
-    mgp.check(L.mgp_tuning_set(b"sweep.rows", rows))
-    yield k, rows
+    for key, v in zip(keys, request.param):
+        mgp.check(L.mgp_tuning_set(key.encode(), v))
+    yield request.param
     for key, v in old.items():
         mgp.check(L.mgp_tuning_set(key.encode(), v))
 
 
-@pytest.mark.parametrize("sweep_kernel", SWEEP_KERNELS, ids=["k%d_r%d" % kr for kr in SWEEP_KERNELS], indirect=True)
+@pytest.mark.parametrize("sweep_kernel", SWEEP_KERNELS, ids=["k%d_r%d_d%d" % kr for kr in SWEEP_KERNELS],
+                         indirect=True)
 @pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
 @pytest.mark.parametrize("shape,bc", SWEEP_SHAPES, ids=[str(i) for i in range(len(SWEEP_SHAPES))])
 def test_one_pass_sweep_equals_oracle_relax(shape, bc, mode, sweep_kernel):
