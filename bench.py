@@ -80,6 +80,11 @@ def main():
                     help="mgp_tuning_set before the run (e.g. sweep.target_wgs=4096)")
     ap.add_argument("--cpu-threads", type=int, default=0, help="0 = all host cores")
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32+FP64-refinement leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batched-solves leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
+    ap.add_argument("--batch-count", type=int, default=256)
+    ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--seed", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the solve on one GPU; --gpus %d is not supported "
@@ -146,10 +151,10 @@ def main():
     resid_gbs = resid_bytes / (resid_ms * 1e-3) / 1e9
     log("bench: sweep %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" % (sweep_ms, sweep_gbs, resid_ms, resid_gbs))
 
-    mixed = None
-    if not args.no_mixed:
-        s.close()
-        mixed = mixed_leg(dims, h, args)
+    s.close()
+    mixed = guarded_leg("mixed", args.no_mixed, lambda: mixed_leg(dims, h, args))
+    batch = guarded_leg("batch", args.no_batch, lambda: batch_leg(args))
+    serve = guarded_leg("serve", args.no_serve, lambda: serve_leg(args))
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -178,6 +183,8 @@ def main():
                      "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
         "kernels": {"residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
         "fp32_refine": mixed,
+        "batched": batch,
+        "serving": serve,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
@@ -250,6 +257,166 @@ def mixed_leg(dims, h, args):
     log("bench: mixed %.2fx FP64, |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" %
         (out["speedup"], d_fp64, err_mixed, err_fp64))
     return out
+
+
+def guarded_leg(name, skip, fn):
+    """A secondary leg; its failure is reported in the line, not fatal."""
+    if skip:
+        return None
+    try:
+        return fn()
+    except Exception as e:  # noqa: BLE001
+        log("bench: %s leg failed: %r" % (name, e))
+        return {"error": repr(e)}
+
+
+def max_levels(d):
+    L = 1
+    while all(((n - 1) >> (L - 1)) % 2 == 0 and ((n - 1) >> (L - 1)) // 2 >= 2 for n in d):
+        L += 1
+    return L
+
+
+def modes_of(rng, count=4):
+    return [(int(a), int(b), int(c), float(rng.standard_normal())) for a, b, c in rng.integers(1, 9, size=(count, 3))]
+
+
+def modes_rhs(d, modes):
+    """Host f = sum A sin(a pi x) sin(b pi y) sin(c pi z), x = i/(nx-1), separable."""
+    import numpy as np
+
+    h = 1.0 / (d[0] - 1)
+    x, y, z = (np.arange(n) * h for n in d)
+    f = np.zeros(d)
+    for a, b, c, amp in modes:
+        f += amp * np.sin(a * np.pi * x)[:, None, None] * (np.sin(b * np.pi * y)[:, None] * np.sin(c * np.pi * z))
+    return f
+
+
+def batch_leg(args):
+    """Config 4: batched FP32-cycle / FP64-refinement solves (Jacobi 6/7, 3+3,
+    V-cycle, FW, trilinear, zero Dirichlet) vs the same problems solved one at
+    a time (mixed and plain FP64, hierarchies resident, a timed sample)."""
+    import numpy as np
+
+    rng = np.random.default_rng(args.seed + 4)
+    n = args.batch_count
+    dims = [tuple(int(v) for v in rng.choice([33, 65, 129], size=3)) for _ in range(n)]
+    modes = [modes_of(rng) for _ in range(n)]
+    dof = sum(a * b * c for a, b, c in dims)
+    log("bench: batch leg, %d problems, %.3g DOF" % (n, dof))
+    b = mgp.Batch(dims, relaxation=mgp.JACOBI, nsmooth=3, omega=6.0 / 7.0, maxiter=60, tolerance=args.tol)
+    for p in range(n):
+        b.set_rhs(p, modes_rhs(dims[p], modes[p]))
+    b.solve()  # records the iteration graphs
+    t = b.solve()
+    st = [b.stats(p) for p in range(n)]
+    worst = max(r for _, r in st)
+    log("bench:   batched %.1f ms, iterations %d..%d, worst rel_res %.2e" %
+        (1e3 * t, min(i for i, _ in st), max(i for i, _ in st), worst))
+    b.close()
+    # one-at-a-time solves of a sample of the same problems
+    k = min(n, 24)
+    seq = {}
+    for name, dt in (("fp32_refine_single", mgp.MIXED), ("fp64_single", mgp.F64)):
+        tot = 0.0
+        for p in range(k):
+            d = dims[p]
+            s = mgp.Solver(d, max_levels(d), 1.0 / (d[0] - 1), (0.0, 0.0, 0.0), "DDDDDD", mgp.CONSISTENT, mgp.JACOBI,
+                           mgp.FULL_WEIGHTING, problem=-1, dtype=dt, maxiter=60, miniter=1, tolerance=args.tol)
+            s.array("f")[:] = modes_rhs(d, modes[p]).ravel()
+            s.to_device()
+            s.solve("Vcycle", 3, 6.0 / 7.0)  # records the cycle graph
+            s.reset()
+            s.sync()
+            t0 = time.perf_counter()
+            s.solve("Vcycle", 3, 6.0 / 7.0)
+            s.sync()
+            tot += time.perf_counter() - t0
+            s.close()
+        seq[name] = {"solves/s": k / tot, "sample": "%d of %d problems, solve() only" % (k, n)}
+    return {"workload": "batched-%d-dirichlet-{33,65,129}^3" % n, "problems": n, "dof": dof, "seconds": t,
+            "solves/s": n / t, "DOF/s": dof / t, "worst_rel_res": worst,
+            "iterations": [min(i for i, _ in st), max(i for i, _ in st)], "one_at_a_time": seq,
+            "speedup_vs_fp64_single": (n / t) / seq["fp64_single"]["solves/s"]}
+
+
+def serve_requests(rng, count):
+    out = []
+    while len(out) < count:
+        d = tuple(int(2 ** k + 1) for k in rng.integers(5, 10, size=3))
+        bc = "".join("N" if v else "D" for v in rng.integers(0, 2, size=6))
+        if "D" not in bc:  # all-Neumann is singular: redrawn
+            continue
+        out.append((d, bc, modes_of(rng)))
+    return out
+
+
+def serve_leg(args):
+    """Config 5: the serving loop (FP32-cycle / FP64-refinement F-cycles,
+    tol 1e-8) on requests of 2^k+1 points per axis (k in 5..9), face types
+    by seed, Fourier right-hand sides.  Capacity from a burst; p50/p99 at
+    Poisson arrivals of 0.75 x that capacity; against one-at-a-time plain
+    FP64 F-cycle solves of the same requests (fresh hierarchy each, as the
+    server's requests of new shapes get)."""
+    import numpy as np
+
+    rng = np.random.default_rng(args.seed + 5)
+    reqs = serve_requests(rng, args.serve_count)
+    tol = 1e-8
+    L = mgp.lib()
+    # one-at-a-time FP64
+    log("bench: serving leg, %d requests; one-at-a-time FP64" % len(reqs))
+    t0 = time.perf_counter()
+    for d, bc, modes in reqs:
+        s = mgp.Solver(d, max_levels(d), 1.0 / (d[0] - 1), (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                       mgp.FULL_WEIGHTING, problem=None, dtype=mgp.F64, maxiter=100, miniter=1, tolerance=tol,
+                       coarse_sweeps=32)
+        mgp.check(L.mgp_solver_to_device_zero(s.h))
+        fa = s.device_array("f")
+        abc = (ctypes.c_int32 * 48)(*[v for a, b_, c, _ in modes for v in (a, b_, c)])
+        amp = (ctypes.c_double * 16)(*[m[3] for m in modes])
+        mgp.check(L.mgp_fill_modes_f64(ctypes.byref(fa), len(modes), abc, amp, 1.0 / (d[0] - 1), None))
+        it, rr, _ = s.solve("Fcycle", 2, 1.0)
+        ua = s.device_array("u")
+        host = np.empty(int(np.prod(mgp.padded_shape(list(ua.shape), list(ua.halo)))))
+        mgp.check(L.mgp_memcpy_d2h(host.ctypes.data, ua.data, host.nbytes, None))
+        s.close()
+    seq_s = time.perf_counter() - t0
+    seq_rate = len(reqs) / seq_s
+    log("bench:   %.2f s, %.1f solves/s" % (seq_s, seq_rate))
+
+    def run(arrival_rate):
+        srv = mgp.Server(hbm_budget=96 << 30, max_active=4, tolerance=tol, coarse_sweeps=32)
+        start = time.perf_counter()
+        ids = []
+        t_next = start
+        arr = np.random.default_rng(args.seed + 55)
+        for d, bc, modes in reqs:
+            if arrival_rate:
+                t_next += arr.exponential(1.0 / arrival_rate)
+                dt = t_next - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+            ids.append(srv.submit(d, bc, modes, want_solution=True))
+        res = [srv.wait(i)[0] for i in ids]
+        wall = time.perf_counter() - start
+        _, peak, resident = srv.stats()
+        srv.close()
+        lat = np.array([r.latency_s for r in res])
+        bad = [r for r in res if r.status != 0 or r.rel_res > tol]
+        return {"solves/s": len(reqs) / wall, "p50_latency_s": float(np.percentile(lat, 50)),
+                "p99_latency_s": float(np.percentile(lat, 99)), "peak_active": peak, "failed": len(bad)}
+
+    log("bench:   server burst")
+    burst = run(None)
+    rate = 0.75 * burst["solves/s"]
+    log("bench:   server at Poisson arrivals %.1f/s" % rate)
+    poisson = run(rate)
+    poisson["arrival_rate"] = rate
+    return {"workload": "serving-%d-requests-2^k+1-mixed-bc" % len(reqs), "tolerance": tol,
+            "one_at_a_time_fp64_solves/s": seq_rate, "burst": burst, "poisson": poisson,
+            "capacity_vs_fp64_one_at_a_time": burst["solves/s"] / seq_rate}
 
 
 def cpu_baseline(dims, levels, h, nsmooth, cycles, threads, coarse_sweeps):

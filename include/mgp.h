@@ -225,6 +225,10 @@ int mgp_solver_init_fn(void* solver, double (*u)(double, double, double), double
                        double (*dudz)(double, double, double));
 int mgp_solver_init_zero(void* solver);  /* libpoisson.h:174-190 */
 int mgp_solver_to_device(void* solver);  /* libpoisson.h:193-209 */
+/* the device state to_device() gives right after creation with every array
+ * zero (init_zero), made on the device without a host transfer (no
+ * counterpart in the reference) */
+int mgp_solver_to_device_zero(void* solver);
 int mgp_solver_to_host(void* solver);    /* libpoisson.h:212-228 */
 /* solve(solver, nsmooth, omega, "", maxtime) (libpoisson.h:237-357);
  * solver = "Vcycle"/"vcycle"/"Fcycle"/"fcycle"/"Relaxation"/"relaxation" */
@@ -345,13 +349,15 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
 
 /* ---------------------------------------------------------------- tuning
  * Launch-shape knobs (process-wide; take effect for the next launches and for
- * graphs recorded afterwards).  Keys: "sweep.target_wgs" (workgroups a
- * one-pass sweep aims for, default 4096), "sweep.min_chunk" (shortest x march
- * per workgroup, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup,
- * default 8), "sweep.kernel" (1: workgroup tiles sharing planes through LDS,
- * 2: one wave per tile, values in registers), "sweep.rows" (8, 12 or 16
- * register rows per wave tile, kernel 2), "sweep.depth" (1 or 2 planes of
- * loads in flight per wave, kernel 2).  No counterpart in the reference, whose launch
+ * graphs recorded afterwards).  Keys: "sweep.kernel" (1: workgroup tiles
+ * sharing planes through LDS; 2: one wave per tile, values in registers,
+ * default), "sweep.target_wgs" (workgroups, or waves for kernel 2, a one-pass
+ * sweep aims for; default 16384), "sweep.min_chunk" (shortest x march per
+ * tile, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup, kernel 1,
+ * default 8), "sweep.rows" (8, 12 or 16 register rows per wave tile, kernel 2;
+ * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
+ * loads in flight per wave, kernel 2, default 1).  No counterpart in the
+ * reference, whose launch
  * shape was fixed by the OpenMP runtime. */
 int mgp_tuning_set(const char* key, int64_t value);
 int mgp_tuning_get(const char* key, int64_t* value);

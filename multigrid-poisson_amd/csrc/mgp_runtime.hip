@@ -117,12 +117,13 @@ int mgp_event_elapsed_ms(void* start, void* stop, float* ms) {
 }
 
 int mgp_tuning_set(const char* key, int64_t value) {
-    if (!key || value < 1) return MGP_ERR_ARG;
-    if (!strcmp(key, "sweep.target_wgs")) mgp::g_sweep_target_wgs = value;
-    else if (!strcmp(key, "sweep.min_chunk")) mgp::g_sweep_min_chunk = value;
+    if (!key || value < 0) return MGP_ERR_ARG;
+    if (!strcmp(key, "sweep.target_wgs") && value >= 1) mgp::g_sweep_target_wgs = value;
+    else if (!strcmp(key, "sweep.min_chunk") && value >= 1) mgp::g_sweep_min_chunk = value;
     else if (!strcmp(key, "sweep.waves") && (value == 4 || value == 8 || value == 16)) mgp::g_sweep_waves = value;
     else if (!strcmp(key, "sweep.kernel") && (value == 1 || value == 2)) mgp::g_sweep_kernel = value;
-    else if (!strcmp(key, "sweep.rows") && (value == 8 || value == 12 || value == 16)) mgp::g_sweep_rows = value;
+    else if (!strcmp(key, "sweep.rows") && (value == 0 || value == 8 || value == 12 || value == 16))
+        mgp::g_sweep_rows = value;
     else if (!strcmp(key, "sweep.depth") && (value == 1 || value == 2)) mgp::g_sweep_depth = value;
     else return MGP_ERR_ARG;
     return MGP_OK;

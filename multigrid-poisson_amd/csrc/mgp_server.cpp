@@ -112,8 +112,7 @@ struct Slot : SlotBase {
         auto t = [&](int q) { return r.bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
         bc = Poisson::BoundaryCondition{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
         solver.reset(new Poisson::RefinementSolver<Poisson::FullWeighting, Poisson::TrilinearInterpolation, S>(st, bc));
-        solver->init_zero();
-        solver->to_device();
+        solver->to_device_zero();  // zero f, u and (zero) face data, on the device
         nsmooth = c.nsmooth;
         omega = c.omega;
         hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "server event");

@@ -19,6 +19,7 @@ struct SolverBase {
                          Poisson::funptr dz) = 0;
     virtual void init_zero() = 0;
     virtual void to_device() = 0;
+    virtual void to_device_zero() = 0;
     virtual void to_host() = 0;
     virtual void solve(const std::string& c, int64_t nsmooth, double omega, double maxtime) = 0;
     virtual void cycle(const std::string& c, int64_t nsmooth, double omega) = 0;
@@ -41,6 +42,7 @@ struct SolverImpl : SolverBase {
     }
     void init_zero() override { s.init_zero(); }
     void to_device() override { s.to_device(); }
+    void to_device_zero() override { s.to_device_zero(); }
     void to_host() override { s.to_host(); }
     void solve(const std::string& c, int64_t nsmooth, double omega, double maxtime) override {
         s.solve(c, nsmooth, (T)omega, "", maxtime);
@@ -91,6 +93,7 @@ struct MixedImpl : SolverBase {
     }
     void init_zero() override { s.init_zero(); }
     void to_device() override { s.to_device(); }
+    void to_device_zero() override { s.to_device_zero(); }
     void to_host() override { s.to_host(); }
     void solve(const std::string& c, int64_t nsmooth, double omega, double maxtime) override {
         s.solve(c, nsmooth, omega, "", maxtime);
@@ -233,6 +236,7 @@ int mgp_solver_init_fn(void* h, double (*u)(double, double, double), double (*f)
 
 int mgp_solver_init_zero(void* h) { return h ? guarded([&] { S(h)->init_zero(); }) : MGP_ERR_ARG; }
 int mgp_solver_to_device(void* h) { return h ? guarded([&] { S(h)->to_device(); }) : MGP_ERR_ARG; }
+int mgp_solver_to_device_zero(void* h) { return h ? guarded([&] { S(h)->to_device_zero(); }) : MGP_ERR_ARG; }
 int mgp_solver_to_host(void* h) { return h ? guarded([&] { S(h)->to_host(); }) : MGP_ERR_ARG; }
 
 int mgp_solver_solve(void* h, const char* cycle, int64_t nsmooth, double omega, double maxtime) {

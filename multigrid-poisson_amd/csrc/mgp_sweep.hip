@@ -632,7 +632,10 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
     // range (a plane must stay below the out-of-range lane offset)
     const int64_t plane_bytes = ((A.ui.n1 - 1) * A.ui.s1 + A.ui.n2) * (int64_t)sizeof(T);
     if (g_sweep_kernel == 2 && A.ui.n0 >= 3 && plane_bytes < (int64_t)BUF_OOB) {
-        const int rows = g_sweep_rows == 8 || g_sweep_rows == 12 ? (int)g_sweep_rows : 16;
+        // register rows per wave tile: measured best 12 (FP64) and 16 (FP32)
+        // at 1025 x 1025 x 513
+        const int rows = g_sweep_rows == 8 || g_sweep_rows == 12 || g_sweep_rows == 16 ? (int)g_sweep_rows
+                                                                                        : (sizeof(T) == 8 ? 12 : 16);
         A.ytiles = (int)((b[3] - b[2] + rows - 3) / (rows - 2));
         A.ztiles = (int)((b[5] - b[4] + WZ - 1) / WZ);
         const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
@@ -673,11 +676,11 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
 }
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
-int64_t g_sweep_target_wgs = 4096;  // workgroups to aim for per launch
+int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch
 int64_t g_sweep_min_chunk = 48;     // shortest x march per workgroup
 int64_t g_sweep_waves = 8;          // waves per workgroup: 4, 8 or 16
-int64_t g_sweep_kernel = 1;         // 1: workgroup tiles with LDS, 2: wave-private tiles
-int64_t g_sweep_rows = 16;          // register rows per wave tile: 8, 12 or 16
+int64_t g_sweep_kernel = 2;         // 1: workgroup tiles with LDS, 2: wave-private tiles
+int64_t g_sweep_rows = 0;           // register rows per wave tile: 8, 12, 16; 0: by type
 int64_t g_sweep_depth = 1;          // planes of loads in flight per wave: 1 or 2
 
 // x-chunk length: enough workgroups to keep every CU busy with several

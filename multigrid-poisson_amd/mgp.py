@@ -90,7 +90,7 @@ _SIGS = {
     "mgp_event_elapsed_ms": [vp, vp, P(f)],
     "mgp_tuning_set": [ctypes.c_char_p, i64], "mgp_tuning_get": [ctypes.c_char_p, P(i64)],
     "mgp_solver_create": [P(SolverConfig), P(vp)], "mgp_solver_destroy": [vp],
-    "mgp_solver_init_problem": [vp, i32], "mgp_solver_init_zero": [vp], "mgp_solver_to_device": [vp],
+    "mgp_solver_init_problem": [vp, i32], "mgp_solver_init_zero": [vp], "mgp_solver_to_device": [vp], "mgp_solver_to_device_zero": [vp],
     "mgp_solver_to_host": [vp], "mgp_solver_solve": [vp, ctypes.c_char_p, i64, d, d],
     "mgp_solver_cycle": [vp, ctypes.c_char_p, i64, d], "mgp_solver_relres": [vp, P(d)], "mgp_solver_sync": [vp], "mgp_solver_reset": [vp],
     "mgp_solver_stats": [vp, P(d), P(d), P(i64)], "mgp_solver_array": [vp, i32, i32, P(vp), P(Array)],

@@ -8,6 +8,7 @@
 #include <stdio.h>
 
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <iostream>
 #include <new>
@@ -78,14 +79,17 @@ Array<T>::Array(uint_t i, uint_t j, uint_t k, Halo& _halo)
 
 template <class T>
 Array<T>::~Array() {
-    delete[] this->at;
+    std::free(this->at);
 }
 
 // Zero-initialised (the reference leaves new T[] uninitialised; a zeroed mirror
-// makes every array well defined before its first write).
+// makes every array well defined before its first write).  calloc: large
+// mirrors come from zero pages the OS maps on first touch, so a mirror that is
+// never used costs no host time.
 template <class T>
 void Array<T>::allocator() {
-    this->at = new T[size]();
+    this->at = static_cast<T*>(std::calloc(size > 0 ? size : 1, sizeof(T)));
+    if (!this->at) throw std::bad_alloc();
 }
 
 template <class T>

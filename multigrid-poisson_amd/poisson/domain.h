@@ -69,6 +69,9 @@ public:
     void init(funptr ufun, funptr ffun, funptr dudxfun, funptr dudyfun, funptr dudzfun);
     void init_zero();
     void to_device();
+    // the device state to_device() gives right after init_zero(), made on the
+    // device (no host transfer; the host mirrors stay untouched)
+    void to_device_zero();
     void to_host();
     void save(std::string ufilename = "u.vtk");
     void save_all(std::string ufilename = "u.vtk", std::string ffilename = "f.vtk", std::string rfilename = "u.vtk");
@@ -218,6 +221,24 @@ void Domain<T>::to_device() {
     if (u_alt)
         mgpx::check(mgp_memcpy_d2d(u_alt, u->devptr, (size_t)u->size * sizeof(T), mgpx::stream()), "Domain::to_device");
     mgpx::check(mgp_stream_sync(mgpx::stream()), "Domain::to_device");
+}
+
+template <class T>
+void Domain<T>::to_device_zero() {
+    DeviceArray<T>* arrays[] = {u, requires_duplicate ? uprev : nullptr, f, r, &north->arr, &south->arr,
+                                &east->arr, &west->arr, &top->arr, &bottom->arr};
+    for (DeviceArray<T>* a : arrays) {
+        if (!a) continue;
+        a->mark_on_device();
+        a->init_zero();
+    }
+    if (u_alt) {
+        const mgp_array d = u->desc();
+        mgp_array alt = d;
+        alt.data = u_alt;
+        mgpx::check(mgpx::Ops<T>::fill(&alt, 0.0, mgpx::stream()), "Domain::to_device_zero");
+    }
+    mgpx::check(mgp_stream_sync(mgpx::stream()), "Domain::to_device_zero");
 }
 
 template <class T>

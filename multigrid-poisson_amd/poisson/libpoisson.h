@@ -98,6 +98,8 @@ public:
     void init();
     void init_zero();
     void to_device();
+    // to_device() of a freshly constructed (all-zero) solver, made on the device
+    void to_device_zero();
     void to_host();
     void verbose(bool onoff);
     void solve(const std::string solver = "Vcycle", const int_t nsmooth = 10, T omega = -1.0, std::string ofile = "",
@@ -227,6 +229,13 @@ void PoissonSolver<T, R, P, S>::init_zero() {
     mgpx::StreamScope scope(stream);
     for (uint_t l = 0; l < settings.levels; l++) domains[0][l]->init_zero();
     mgpx::check(mgp_stream_sync(stream), "init_zero");
+}
+
+template <class T, template <class> class R, template <class> class P, template <class> class S>
+void PoissonSolver<T, R, P, S>::to_device_zero() {
+    mgpx::StreamScope scope(stream);
+    clear_graphs();
+    for (uint_t l = 0; l < settings.levels; l++) domains[0][l]->to_device_zero();
 }
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>

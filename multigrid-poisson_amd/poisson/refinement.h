@@ -74,6 +74,11 @@ public:
         inner->to_device();
     }
     void to_host() { outer->to_host(); }
+    // to_device() of a freshly constructed (all-zero) solver, made on the device
+    void to_device_zero() {
+        outer->to_device_zero();
+        inner->to_device_zero();
+    }
     void verbose(bool onoff) { is_verbose = onoff; }
     void reset_solution() { outer->reset_solution(); }
 
