@@ -85,6 +85,9 @@ def main():
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
     ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--traffic-probe", action="store_true",
+                    help="run once each: a fill (write-only) and an inf-norm (read-only) of one 4.3 GB array, "
+                         "then one fine-level one-pass sweep of the workload; no JSON line (for rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the solve on one GPU; --gpus %d is not supported "
@@ -92,6 +95,9 @@ def main():
         sys.exit(2)
 
     mgp.require_gpu()
+    if args.traffic_probe:
+        traffic_probe(args)
+        return
     for kv in args.tune:
         k, v = kv.split("=")
         mgp.check(mgp.lib().mgp_tuning_set(k.encode(), int(v)), "mgp_tuning_set")
@@ -179,8 +185,8 @@ def main():
                    "coarsest_sweeps": args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "rbgs_sweep (fine level)", "achieved": sweep_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS, "traffic": None,
-                     "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(n), "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
         "kernels": {"residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
         "fp32_refine": mixed,
         "batched": batch,
@@ -188,6 +194,20 @@ def main():
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def measured_traffic(n):
+    """HBM bytes per fine-level sweep launch from the rocprofv3 FETCH_SIZE /
+    WRITE_SIZE runs summarised in profiles/ (scaled by calibration kernels of
+    known size), for the 1025x1025x513 workload; None for other sizes."""
+    path = os.path.join(ROOT, "profiles", "rbgs_sweep_%dx%dx%d_traffic.md" % (n, n, n // 2 + 1))
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":")[1])
+    except OSError:
+        pass
+    return None
 
 
 def timed_solves(s, args):
@@ -257,6 +277,38 @@ def mixed_leg(dims, h, args):
     log("bench: mixed %.2fx FP64, |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" %
         (out["speedup"], d_fp64, err_mixed, err_fp64))
     return out
+
+
+def traffic_probe(args):
+    """Three kernels for HBM-counter calibration: fill of u (8 B/point written),
+    inf-norm of f (8 B/point read), one one-pass sweep u -> r (24 B/box point)."""
+    import numpy as np
+
+    n, nz = args.n, args.n // 2 + 1
+    h = 1.0 / (n - 1)
+    s = mgp.Solver((n, n, nz), args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET | mgp.NO_GRAPHS, coarse_sweeps=args.coarse_sweeps)
+    s.to_device()
+    L = mgp.lib()
+    u, f_, r = s.device_array("u"), s.device_array("f"), s.device_array("r")
+    scal = mgp.DeviceBuffer(np.zeros(1))
+    F = mgp.Faces()
+    for q in range(6):
+        t = s.face_type(q)
+        F.type[q] = t
+        host, a, tt = mgp.vp(), mgp.Array(), ctypes.c_int()
+        mgp.check(L.mgp_solver_face(s.h, 0, q, ctypes.byref(host), ctypes.byref(a), ctypes.byref(tt)))
+        F.g[q] = a.data if t == mgp.NEUMANN else None
+    F.h = h
+    L.mgp_solver_sync(s.h)
+    mgp.check(L.mgp_fill_f64(ctypes.byref(r), 0.0, None))
+    mgp.check(L.mgp_norm_inf_f64(ctypes.byref(f_), scal.ptr, None))
+    mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_), ctypes.byref(F),
+                                   mgp.box([0, n, 0, n, 1, nz - 1]), 1.0, None))
+    L.mgp_stream_sync(None)
+    pts = n * n * nz
+    log("traffic probe: fill %d B, norm %d B, sweep %d B algorithmic" % (8 * pts, 8 * pts, 24 * n * n * (nz - 2)))
+    s.close()
 
 
 def guarded_leg(name, skip, fn):

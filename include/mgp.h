@@ -253,6 +253,14 @@ int mgp_solver_save(void* solver, const char* path);
 /* message of the last failing mgp_solver_* / mgp_batch_* call on this thread */
 const char* mgp_last_error(void);
 
+/* residual of FP64 u and f on the box, written as FP32 into `out` (same
+ * shape, halos and element strides as u) instead of r; with norm, max |r|
+ * over the box and r's stored values outside it.  r itself is only read
+ * (outside the box).  The residual and cast of mixed-precision refinement in
+ * one pass (no counterpart in the reference). */
+int mgp_residual_cast_f64_f32(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                              const mgp_array* out, const int64_t box[6], double h, double* norm, void* stream);
+
 /* synthetic right-hand side over the shape of f: sum over nmodes (<= 16) of
  * amp[m] sin(a pi x) sin(b pi y) sin(c pi z), (a, b, c) = abc[3m..3m+2], at
  * x = i h, y = j h, z = k h.  No counterpart in the reference (its f comes
@@ -338,6 +346,8 @@ typedef struct {
     double latency_s;                  /* submit -> result on the host */
     double queue_s;                    /* submit -> admission */
     double run_s;                      /* admission -> result on the host */
+    double setup_s;                    /* admission: slot (hierarchy) creation or reuse, f, u = 0 */
+    double fetch_s;                    /* solution copy to the host */
 } mgp_result;
 int mgp_server_create(const mgp_server_config* config, void** server);
 /* waits for every submitted request to finish */
@@ -356,7 +366,10 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * tile, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup, kernel 1,
  * default 8), "sweep.rows" (8, 12 or 16 register rows per wave tile, kernel 2;
  * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
- * loads in flight per wave, kernel 2, default 1).  No counterpart in the
+ * loads in flight per wave, kernel 2, default 1), "residual.kernel" (1:
+ * per-point kernels; 2: one wave per tile marching along x, default),
+ * "residual.rows" (8 or 16 register rows per residual tile, default 16).
+ * No counterpart in the
  * reference, whose launch
  * shape was fixed by the OpenMP runtime. */
 int mgp_tuning_set(const char* key, int64_t value);

@@ -501,6 +501,7 @@ static int residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_
         !valid_box(u, b) || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
         return MGP_ERR_ARG;
     const Box bx = box_of(b);
+    if (g_resid_kernel == 2 && march_residual<T>(mode, u, f, r, b, h, norm, stream) == MGP_OK) return MGP_OK;
     const T hsq = (T)(h * h);  // include/residual.h:23
     const T inv_h2 = (T)(1.0 / (h * h));
     const hipStream_t s = (hipStream_t)stream;
@@ -686,6 +687,15 @@ int mgp_fill_modes_f64(const mgp_array* f, int64_t nmodes, const int32_t* abc, c
     hipLaunchKernelGGL(k_fill_modes<double>, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*f), bx, g.zchunks,
                        m, h);
     return launch_status();
+}
+
+int mgp_residual_cast_f64_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                              const mgp_array* out, const int64_t b[6], double h, double* norm, void* s) {
+    if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !valid_array(out) || !same_shape(u, f) ||
+        !same_shape(u, r) || !same_shape(u, out) || !valid_box(u, b) ||
+        (m != MGP_MODE_REFERENCE && m != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    return march_residual_to<double, float>(m, u, f, r, out, b, h, norm, s);
 }
 
 int mgp_zero_scalars(double* p, int64_t n, void* s) {

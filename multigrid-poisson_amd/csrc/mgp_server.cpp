@@ -271,7 +271,9 @@ private:
         Job* j = s.job;
         if (status == MGP_OK) {
             try {
+                const auto t0 = Clock::now();
                 s.finish(j->req.u_out);
+                j->res.fetch_s = seconds(t0, Clock::now());
                 s.stats(&j->res.iterations, &j->res.rel_res);
             } catch (const std::exception& e) {
                 mgp::set_last_error(e.what());
@@ -304,6 +306,7 @@ private:
                     j = pending.front();
                 }
                 std::unique_ptr<SlotBase> s;
+                j->t_admit = Clock::now();
                 try {
                     s = acquire(j->req);
                 } catch (const std::exception& e) {
@@ -313,7 +316,6 @@ private:
                         std::lock_guard<std::mutex> lk(mu);
                         pending.pop_front();
                     }
-                    j->t_admit = Clock::now();
                     complete_job(j, MGP_ERR_STATE);
                     continue;
                 }
@@ -322,10 +324,10 @@ private:
                     std::lock_guard<std::mutex> lk(mu);
                     pending.pop_front();
                 }
-                j->t_admit = Clock::now();
                 s->job = j;
                 try {
                     s->begin(j->req);
+                    j->res.setup_s = seconds(j->t_admit, Clock::now());
                     s->enqueue();
                 } catch (const std::exception& e) {
                     mgp::set_last_error(e.what());

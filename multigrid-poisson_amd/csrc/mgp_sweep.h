@@ -11,6 +11,17 @@ namespace mgp {
 template <class T>
 int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
                 const int64_t box[6], double omega, int64_t xchunk, void* stream);
+// residual on a box through wave tiles (norm: max |r| over the whole shape
+// folded into *norm); MGP_ERR_ARG when the layouts do not allow it
+template <class T>
+int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
+                   double h, double* norm, void* stream);
+// the same with r written, in type O, to `out` (same element layout) instead of r
+template <class T, class O>
+int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
+                      const int64_t box[6], double h, double* norm, void* stream);
+extern int64_t g_resid_kernel;
+extern int64_t g_resid_rows;
 // default x-chunk length for a box
 int64_t march_xchunk(const int64_t box[6]);
 extern int64_t g_sweep_target_wgs;

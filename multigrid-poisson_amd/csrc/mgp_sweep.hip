@@ -601,6 +601,135 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
     else wave_march<T, MODE, R, D, false>(A, tl);
 }
 
+
+// ---------------------------------------------------------------------------
+// Residual r = f - L u on a box (include/residual.h:58-79), one wave per
+// (y, z) tile marching along x like the sweep: register rows for y, DPP lane
+// shifts for z, the previous and next planes' registers for x.  The ghosts
+// are read from the halo, which the caller has refreshed
+// (include/residual.h:26-34).  With `norm`, max |r| over the box is folded into
+// *norm (atomic max on the bits, exact); the caller adds the shape points
+// outside the box.
+template <class T, class O = T>
+struct ResidArgs {
+    View<T> u, f, r;
+    View<O> out;  // where r goes (r itself, or a narrower copy)
+    int64_t bx0, bx1, by0, by1, bz0, bz1;
+    int64_t ylo, yhi, zlo, zhi, xlo, xhi;  // allocated (halo-inclusive) index ranges
+    T hsq, inv_h2;
+    unsigned long long* norm;
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t xchunk;
+};
+
+constexpr int RZ = 62;  // output lanes per residual tile: lanes 1..62
+
+template <class T, int MODE>
+__device__ __forceinline__ T resid_point(T uc, T xm, T xp, T ym, T yp, T zm, T zp, T fc, T hsq, T inv_h2) {
+    if (MODE == MGP_MODE_REFERENCE) {
+        return fc - (xm + xp + ym + yp + zm + zp - 6.0 * uc) / hsq;  // residual.h:70-72, double literal
+    } else {
+        const T s = ((xm - uc) + (xp - uc)) + ((ym - uc) + (yp - uc)) + ((zm - uc) + (zp - uc));
+        return fma(-inv_h2, s, fc);
+    }
+}
+
+template <class T, class O, int MODE, int R, bool NORM>
+__global__ void __launch_bounds__(64) k_residual_wave(ResidArgs<T, O> A) {
+    constexpr int NR = R + 2;
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * R;
+    const int64_t z0 = A.bz0 + (int64_t)tz * RZ;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+    const int64_t s0 = A.u.s0, s1 = A.u.s1;
+    // buffer base: (plane, ylo, zlo); every offset below is in range
+    const uint32_t plane_bytes = (uint32_t)(((A.yhi - A.ylo) * s1 + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(T));
+    const int lane = threadIdx.x;
+    const int64_t z = z0 - 1 + lane;
+    const bool lane_out = lane >= 1 && lane <= RZ && z < A.bz1;
+    const int64_t zc = min(max(z, A.zlo), A.zhi);
+    const uint32_t zoff = (uint32_t)((zc - A.zlo) * (int64_t)sizeof(T));
+    uint32_t roff[NR];
+    bool yout[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int64_t y = y0 - 1 + t;
+        roff[t] = (uint32_t)((min(max(y, A.ylo), A.yhi) - A.ylo) * s1 * (int64_t)sizeof(T));
+        yout[t] = t >= 1 && t <= R && y < A.by1;
+    }
+    const int64_t corner = A.ylo * s1 + A.zlo;  // element offset of (0, ylo, zlo) from (0, 0, 0)
+    auto rsrc = [&](const T* p, int64_t i) {
+        return plane_rsrc(p + min(max(i, A.xlo), A.xhi) * s0 + corner, plane_bytes);
+    };
+    auto load_u = [&](int64_t i, T (&U)[NR]) {
+        const __amdgpu_buffer_rsrc_t rs = rsrc(A.u.p, i);
+#pragma unroll
+        for (int t = 0; t < NR; t++) U[t] = bload<T>(rs, zoff, roff[t]);
+    };
+    auto load_f = [&](int64_t i, T (&F)[NR]) {
+        const __amdgpu_buffer_rsrc_t rs = rsrc(A.f.p, i);
+#pragma unroll
+        for (int t = 1; t <= R; t++) F[t] = bload<T>(rs, zoff, roff[t]);
+    };
+    T Um1[NR], U0[NR], Up1[NR], UL[NR], F0[NR], FL[NR];
+    load_u(xa - 1, Um1);
+    load_u(xa, U0);
+    load_u(xa + 1, Up1);
+    load_f(xa, F0);
+    double vmax = 0.0;
+    for (int64_t i = xa; i < xb; i++) {
+        load_u(i + 2, UL);
+        load_f(i + 1, FL);
+        // output plane (element offsets as u's; O may be narrower than T)
+        const __amdgpu_buffer_rsrc_t ro = plane_rsrc(
+            A.out.p + min(max(i, A.xlo), A.xhi) * s0 + corner,
+            (uint32_t)((uint64_t)plane_bytes * sizeof(O) / sizeof(T)));
+#pragma unroll
+        for (int t = 1; t <= R; t++) {
+            const T rv = resid_point<T, MODE>(U0[t], Um1[t], Up1[t], U0[t - 1], U0[t + 1], lane_prev(U0[t]),
+                                              lane_next(U0[t]), F0[t], A.hsq, A.inv_h2);
+            const bool out = lane_out && yout[t];
+            bstore((O)rv, ro, out ? (uint32_t)((uint64_t)zoff * sizeof(O) / sizeof(T)) : BUF_OOB,
+                   (uint32_t)((uint64_t)roff[t] * sizeof(O) / sizeof(T)));
+            if (NORM) vmax = out ? fmax(vmax, fabs((double)rv)) : vmax;
+        }
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            Um1[t] = U0[t];
+            U0[t] = Up1[t];
+            Up1[t] = UL[t];
+            F0[t] = FL[t];
+        }
+    }
+    if (NORM) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+        if (lane == 0 && vmax > 0.0) atomicMax(A.norm, (unsigned long long)__double_as_longlong(vmax));
+    }
+}
+
+// max |r| over a region (the part of the shape outside the box), into *norm
+template <class T>
+__global__ void k_norm_region(View<T> r, int64_t x0, int64_t x1, int64_t y0, int64_t y1, int64_t z0, int64_t z1,
+                              unsigned long long* norm) {
+    const int64_t ny = y1 - y0, nz = z1 - z0;
+    const int64_t n = (x1 - x0) * ny * nz;
+    double v = 0.0;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t i = x0 + t / (ny * nz), j = y0 + (t / nz) % ny, k = z0 + t % nz;
+        v = fmax(v, fabs((double)r.at(i, j, k)));
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+    if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(norm, (unsigned long long)__double_as_longlong(v));
+}
+
 }  // namespace
 
 template <class T>
@@ -674,6 +803,92 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? MGP_OK : (int)e;
 }
+
+
+// Residual through the wave kernel; MGP_ERR_ARG when the layout does not
+// allow it.  out: where r goes on the box (r itself, or an array of type O
+// with the same element layout); the norm reads r outside the box.
+template <class T, class O>
+int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
+                      const int64_t b[6], double h, double* norm, void* stream) {
+    for (int d = 0; d < 3; d++)
+        if (u->stride[d] != f->stride[d] || u->stride[d] != r->stride[d] || u->stride[d] != out->stride[d] ||
+            u->shape[d] != out->shape[d])
+            return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (u->halo[q] != f->halo[q] || u->halo[q] != r->halo[q] || u->halo[q] != out->halo[q]) return MGP_ERR_ARG;
+    ResidArgs<T, O> A;
+    A.u = view_of<T>(*u);
+    A.f = view_of<T>(*f);
+    A.r = view_of<T>(*r);
+    A.out = view_of<O>(*out);
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.xlo = -u->halo[MGP_WEST], A.xhi = u->shape[0] - 1 + u->halo[MGP_EAST];
+    A.ylo = -u->halo[MGP_SOUTH], A.yhi = u->shape[1] - 1 + u->halo[MGP_NORTH];
+    A.zlo = -u->halo[MGP_BOTTOM], A.zhi = u->shape[2] - 1 + u->halo[MGP_TOP];
+    const int64_t plane_bytes = ((A.yhi - A.ylo) * u->stride[1] + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(T);
+    if (plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    A.hsq = (T)(h * h);  // include/residual.h:23
+    A.inv_h2 = (T)(1.0 / (h * h));
+    A.norm = (unsigned long long*)norm;
+    const int rows = g_resid_rows == 8 || g_resid_rows == 16 ? (int)g_resid_rows : 16;
+    const int64_t nx = b[1] - b[0], ny = b[3] - b[2], nz = b[5] - b[4];
+    const hipStream_t st = (hipStream_t)stream;
+    if (nx > 0 && ny > 0 && nz > 0) {
+        A.ytiles = (int)((ny + rows - 1) / rows);
+        A.ztiles = (int)((nz + RZ - 1) / RZ);
+        const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+        int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        A.xchunk = (nx + chunks - 1) / chunks;
+        A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
+        A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+        const dim3 g(A.ntiles_pad), blk(64);
+        const bool ref = mode == MGP_MODE_REFERENCE;
+#define MGP_RES_LAUNCH(RR, NN)                                                                             \
+    if (ref) hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_REFERENCE, RR, NN>), g, blk, 0, st, A); \
+    else hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_CONSISTENT, RR, NN>), g, blk, 0, st, A);
+        if (norm) {
+            if (rows == 8) { MGP_RES_LAUNCH(8, true) } else { MGP_RES_LAUNCH(16, true) }
+        } else {
+            if (rows == 8) { MGP_RES_LAUNCH(8, false) } else { MGP_RES_LAUNCH(16, false) }
+        }
+#undef MGP_RES_LAUNCH
+    }
+    if (norm) {
+        // the shape points outside the box, whose r is read as stored
+        const int64_t n0 = u->shape[0], n1 = u->shape[1], n2 = u->shape[2];
+        struct Reg { int64_t x0, x1, y0, y1, z0, z1; };
+        const Reg regs[6] = {{0, b[0], 0, n1, 0, n2},       {b[1], n0, 0, n1, 0, n2},
+                             {b[0], b[1], 0, b[2], 0, n2},  {b[0], b[1], b[3], n1, 0, n2},
+                             {b[0], b[1], b[2], b[3], 0, b[4]}, {b[0], b[1], b[2], b[3], b[5], n2}};
+        for (const Reg& g : regs) {
+            const int64_t cnt = (g.x1 - g.x0) * (g.y1 - g.y0) * (g.z1 - g.z0);
+            if (g.x1 <= g.x0 || g.y1 <= g.y0 || g.z1 <= g.z0) continue;
+            const int blocks = (int)std::min<int64_t>(1024, (cnt + 255) / 256);
+            hipLaunchKernelGGL(k_norm_region<T>, dim3(blocks), dim3(256), 0, st, A.r, g.x0, g.x1, g.y0, g.y1, g.z0,
+                               g.z1, (unsigned long long*)norm);
+        }
+    }
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+
+template <class T>
+int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6],
+                   double h, double* norm, void* stream) {
+    return march_residual_to<T, T>(mode, u, f, r, r, b, h, norm, stream);
+}
+
+int64_t g_resid_kernel = 2;  // 1: per-point kernels, 2: wave tiles
+int64_t g_resid_rows = 16;   // register rows per residual wave tile: 8 or 16
+
+template int march_residual<double>(int, const mgp_array*, const mgp_array*, const mgp_array*, const int64_t[6],
+                                    double, double*, void*);
+template int march_residual<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const int64_t[6],
+                                   double, double*, void*);
+template int march_residual_to<double, float>(int, const mgp_array*, const mgp_array*, const mgp_array*,
+                                              const mgp_array*, const int64_t[6], double, double*, void*);
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
 int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch

@@ -69,7 +69,8 @@ class Request(ctypes.Structure):
 
 class Result(ctypes.Structure):
     _fields_ = [("status", ctypes.c_int32), ("iterations", ctypes.c_int64), ("rel_res", ctypes.c_double),
-                ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double), ("run_s", ctypes.c_double)]
+                ("latency_s", ctypes.c_double), ("queue_s", ctypes.c_double), ("run_s", ctypes.c_double),
+                ("setup_s", ctypes.c_double), ("fetch_s", ctypes.c_double)]
 
 _lib = None
 P = ctypes.POINTER
@@ -96,6 +97,7 @@ _SIGS = {
     "mgp_solver_stats": [vp, P(d), P(d), P(i64)], "mgp_solver_array": [vp, i32, i32, P(vp), P(Array)],
     "mgp_solver_face": [vp, i32, i32, P(vp), P(Array), P(ctypes.c_int)], "mgp_solver_save": [vp, ctypes.c_char_p],
     "mgp_fill_modes_f64": [P(Array), i64, P(ctypes.c_int32), P(d), d, vp],
+    "mgp_residual_cast_f64_f32": [i32, P(Array), P(Array), P(Array), P(Array), BOX, d, vp, vp],
     "mgp_batch_create": [P(BatchConfig), P(i64), P(vp)], "mgp_batch_destroy": [vp],
     "mgp_batch_set_rhs": [vp, i64, P(d)], "mgp_batch_solve": [vp, P(d)], "mgp_batch_stats": [vp, i64, P(d), P(i64)],
     "mgp_batch_solution": [vp, i64, P(d)], "mgp_batch_tile": [vp, i64, i64, P(i64)],

@@ -99,6 +99,10 @@ public:
                               << host_norm[0] / fnorm << std::endl;
                 if (iteration_done(since() > maxtime)) break;
             }
+            // the iterations wrote r only as FP32: the FP64 r of the final u
+            mgpx::StreamScope scope(outer->get_stream());
+            residual<double>(outer->get_domain(0, 0), true);
+            mgpx::check(mgp_stream_sync(outer->get_stream()), "refinement");
         }
         wtime = since();
     }
@@ -121,21 +125,27 @@ public:
         mgpx::check(mgp_stream_sync(st), "refinement: norm");
         rel_res = host_norm[0] / fnorm;
         iter = 0;
+        f32_ready = false;
     }
     // f32 = r, e = 0, one FP32 cycle, u += e, then the FP64 residual and its
-    // norm (copied to host memory the stream writes when it gets there)
+    // norm (copied to host memory the stream writes when it gets there).  The
+    // residual goes straight to the FP32 right-hand side of the next
+    // iteration ((float) of the same FP64 value a separate cast would read).
     void enqueue_iteration(int_t nsmooth, double omega) {
         void* st = outer->get_stream();
         mgpx::StreamScope scope(st);
         Domain<double>& d64 = outer->get_domain(0, 0);
         Domain<float>& d32 = inner->get_domain(0, 0);
         const mgp_array r64 = d64.r->desc(), f32 = d32.f->desc(), u64 = d64.u->desc();
-        mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
+        if (!f32_ready) mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
         d32.u->init_zero();
         inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
         const mgp_array e32 = d32.u->desc();
         mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
-        enqueue_norm(d64, st);
+        mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
+        residual_norm_to_f32(d64, f32, dev_norm);
+        mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
+        f32_ready = true;
     }
     // stop rule of solve(); `timed_out` stands for the wall-time limit
     bool iteration_done(bool timed_out = false) {
@@ -179,6 +189,7 @@ private:
     double* host_norm = nullptr;
     bool is_verbose = false;
     bool fcycle = true;
+    bool f32_ready = false;  // the FP32 right-hand side already holds (float) r
     uint_t iter = 0;
     double rel_res = 0.0, wtime = 0.0, fnorm = 1.0;
 };

@@ -39,5 +39,17 @@ void residual_norm(Domain<T>& domain, double* norm_dev, bool fetch_neighbor = tr
     mgpx::check(mgpx::Ops<T>::residual(domain.settings.mode, &u, &f, &r, b, domain.settings.h, norm_dev, mgpx::stream()),
                 "residual");
 }
+// addition: the same residual on an FP64 level, written as FP32 into `out`
+// (same layout) instead of r, with max |r| into *norm_dev; r is only read
+// (outside the box, for the norm)
+inline void residual_norm_to_f32(Domain<double>& domain, const mgp_array& out, double* norm_dev) {
+    detail::refresh_boundaries(domain, true);
+    int64_t b[6];
+    domain.box(b);
+    const mgp_array u = domain.u->desc(), f = domain.f->desc(), r = domain.r->desc();
+    mgpx::check(mgp_residual_cast_f64_f32(domain.settings.mode, &u, &f, &r, &out, b, domain.settings.h, norm_dev,
+                                          mgpx::stream()),
+                "residual");
+}
 }  // namespace Poisson
 #endif

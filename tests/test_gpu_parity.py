@@ -402,3 +402,43 @@ def test_mixed_precision_refinement_reaches_fp64_solution(case):
     s.reset()
     it2, rel2, _ = s.solve(cyc, 2, 1.0)
     assert (it2, rel2) == (it, rel)
+
+
+RESID_SHAPES = [((40, 33, 130), "NNNNDD"), ((9, 17, 65), "DDDDDD"), ((33, 30, 64), "NDDNND"), ((17, 15, 129), "DNNDDN"),
+                ((5, 5, 5), "DNDNDN"), ((65, 19, 70), "NNNNNN")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
+@pytest.mark.parametrize("shape,bc", RESID_SHAPES, ids=[str(i) for i in range(len(RESID_SHAPES))])
+def test_wave_residual_equals_point_residual(shape, bc, mode, dtype):
+    """The residual through wave tiles (residual.kernel 2) == the per-point
+    kernel (residual.kernel 1) bit for bit, r and max |r| over the shape, on
+    random u (halo included), f and r, for boxes that touch and that skip
+    each face."""
+    say("case resid", shape, bc, mode, dtype.__name__)
+    L = mgp.lib()
+    halo = [1 if c == "N" else 0 for c in bc]
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc, mode)).encode()))
+    u = mgp.DeviceArray(shape, halo, dtype=dtype)
+    f = mgp.DeviceArray(shape, halo, dtype=dtype)
+    u.upload(rng.standard_normal(u.pshape))
+    f.upload(rng.standard_normal(f.pshape))
+    r0 = rng.standard_normal(f.pshape)
+    box = [int(bc[3] == "D"), shape[0] - int(bc[2] == "D"), int(bc[1] == "D"), shape[1] - int(bc[0] == "D"),
+           int(bc[5] == "D"), shape[2] - int(bc[4] == "D")]
+    h = 1.0 / (shape[0] - 1)
+    sfx = "f64" if dtype == np.float64 else "f32"
+    out = {}
+    for k in (1, 2):
+        mgp.check(L.mgp_tuning_set(b"residual.kernel", k))
+        r = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+        nrm = mgp.DeviceBuffer(np.zeros(1))
+        mgp.call("mgp_residual_" + sfx, mode, u.ref(), f.ref(), r.ref(), mgp.box(box), h, nrm.ptr, None)
+        r2 = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+        mgp.call("mgp_residual_" + sfx, mode, u.ref(), f.ref(), r2.ref(), mgp.box(box), h, None, None)
+        out[k] = (r.download(), nrm.download()[0], r2.download())
+    mgp.check(L.mgp_tuning_set(b"residual.kernel", 2))
+    np.testing.assert_array_equal(out[1][0], out[2][0])
+    np.testing.assert_array_equal(out[1][2], out[2][2])
+    assert out[1][1] == out[2][1] and out[1][1] > 0
