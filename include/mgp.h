@@ -120,6 +120,15 @@ int mgp_rbgs_sweep_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, 
                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 int mgp_rbgs_sweep_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+/* nsweeps such sweeps in one launch, for a level small enough for one
+ * workgroup's LDS ((nx+2)(ny+2)(nz+2) + nx ny nz elements <= 64 KiB;
+ * MGP_ERR_ARG otherwise): the result of the last sweep goes to u_out, which may
+ * be u_in.  Same values as nsweeps calls of mgp_rbgs_sweep_* (the coarsest
+ * level's smoothing, where launch latency dominates). */
+int mgp_rbgs_sweeps_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                        const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
+int mgp_rbgs_sweeps_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                        const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
 /* Weighted Jacobi over the box: u = (1-w)v + w/6 (sum_nb v - h^2 f)
  * (include/jacobi.h:39-82).  v's halos must hold current ghost values. */
 int mgp_jacobi_f64(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t box[6],
@@ -363,7 +372,7 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * sharing planes through LDS; 2: one wave per tile, values in registers,
  * default), "sweep.target_wgs" (workgroups, or waves for kernel 2, a one-pass
  * sweep aims for; default 16384), "sweep.min_chunk" (shortest x march per
- * tile, default 48), "sweep.waves" (4, 8 or 16 waves per workgroup, kernel 1,
+ * tile, default 4), "sweep.waves" (4, 8 or 16 waves per workgroup, kernel 1,
  * default 8), "sweep.rows" (8, 12 or 16 register rows per wave tile, kernel 2;
  * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
  * loads in flight per wave, kernel 2, default 1), "residual.kernel" (1:

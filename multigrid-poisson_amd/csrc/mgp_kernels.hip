@@ -185,6 +185,101 @@ __global__ void k_gs_half(View<T> u, View<T> f, Box bx, int64_t zc, T one_omega,
                                 one_omega, omega_sixth, hsq);
 }
 
+// nsweeps one-pass red-black sweeps of a level that fits in one workgroup's
+// LDS, in one launch: the padded u (halos included) and f are staged in LDS;
+// each sweep is ghost refresh, red half-sweep, ghost refresh, black
+// half-sweep (include/relaxation.h:25-90), with barriers between, exactly
+// what the per-sweep kernels compute; the box points are written to uo at
+// the end.  Latency, not bandwidth, bounds the coarsest levels, and one
+// launch replaces 2 nsweeps of them.
+template <class T, int MODE>
+__global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<T> f, Faces<T> F, Box bx,
+                                                    int64_t nsweeps, T one_omega, T omega_sixth, T hsq,
+                                                    int64_t lin0) {
+    constexpr int MAXP = 8;  // box points per thread: <= 8192 / 1024 for any level that fits
+    extern __shared__ unsigned char lds_raw[];
+    T* su = reinterpret_cast<T*>(lds_raw);
+    // sizes below fit 32-bit indices (the level fits in 64 KiB)
+    const int n0 = (int)ui.n0, n1 = (int)ui.n1, n2 = (int)ui.n2;
+    // LDS layout: the shape with one ghost layer on every side, then f over
+    // the shape.  Ghost cells are written by the refresh before every
+    // half-sweep (Neumann faces) or never read (Dirichlet faces: the box
+    // stops one short of them), so only the shape is staged.
+    const int q2 = n2 + 2, q1 = n1 + 2;
+    const int P = (n0 + 2) * q1 * q2;
+    T* sf = su + P;
+    auto L = [&](int i, int j, int k) { return ((i + 1) * q1 + (j + 1)) * q2 + (k + 1); };
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < P; t += nt) su[t] = (T)0;
+    __syncthreads();
+    for (int t = tid; t < n0 * n1 * n2; t += nt) {
+        const int i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
+        su[L(i, j, k)] = ui.at(i, j, k);
+        sf[t] = f.at(i, j, k);
+    }
+    // this thread's box points: LDS index, f index, colour
+    const int bn0 = (int)(bx.x1 - bx.x0), bn1 = (int)(bx.y1 - bx.y0), bn2 = (int)(bx.z1 - bx.z0);
+    const int nbox = bn0 * bn1 * bn2;
+    int pc[MAXP], pf[MAXP], pcol[MAXP];
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        const int t = tid + m * nt;
+        pc[m] = -1;
+        pf[m] = 0;
+        pcol[m] = 2;
+        if (t < nbox) {
+            const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
+            pc[m] = L(i, j, k);
+            pf[m] = (i * n1 + j) * n2 + k;
+            pcol[m] = colour_of<MODE>(lin0, ui.s0, ui.s1, i, j, k);
+        }
+    }
+    __syncthreads();
+    const int s1l = q2, s0l = q1 * q2;
+    for (int64_t s = 0; s < nsweeps; s++) {
+        for (int colour = 0; colour < 2; colour++) {
+            // Neumann ghosts: sign*2h*g + mirror (include/neumann.h:112-130)
+            for (int q = 0; q < 6; q++) {
+                if (!F.neumann[q]) continue;
+                const int a0 = (q == MGP_EAST || q == MGP_WEST) ? 1 : n0;
+                const int a1 = (q == MGP_NORTH || q == MGP_SOUTH) ? 1 : n1;
+                const int a2 = (q == MGP_TOP || q == MGP_BOTTOM) ? 1 : n2;
+                for (int t = tid; t < a0 * a1 * a2; t += nt) {
+                    const int i = t / (a1 * a2), j = (t / a2) % a1, k = t % a2;
+                    int gi = i, gj = j, gk = k, d = 0;
+                    T sign = (T)1.0;
+                    switch (q) {
+                    case MGP_EAST: gi = n0; d = -2 * s0l; break;
+                    case MGP_WEST: gi = -1; d = 2 * s0l; sign = (T)-1.0; break;
+                    case MGP_NORTH: gj = n1; d = -2 * s1l; break;
+                    case MGP_SOUTH: gj = -1; d = 2 * s1l; sign = (T)-1.0; break;
+                    case MGP_TOP: gk = n2; d = -2; break;
+                    default: gk = -1; d = 2; sign = (T)-1.0; break;
+                    }
+                    const int c = L(gi, gj, gk);
+                    su[c] = sign * F.two_h * F.g[q][t] + su[c + d];
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int m = 0; m < MAXP; m++) {
+                if (pcol[m] != colour) continue;
+                const int c = pc[m];
+                su[c] = relax_point<T, MODE>(su[c], su[c - s0l], su[c + s0l], su[c - s1l], su[c + s1l], su[c - 1],
+                                             su[c + 1], sf[pf[m]], one_omega, omega_sixth, hsq);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        if (pc[m] < 0) continue;
+        const int t = tid + m * nt;
+        const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
+        uo.at(i, j, k) = su[pc[m]];
+    }
+}
+
 // include/jacobi.h:39-82: u = J(v), out of place; the reference writes
 // u = (1-w)v and then adds, which is the same two roundings
 template <class T, int MODE>
@@ -473,6 +568,54 @@ static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const 
     return march_sweep<T>(mode, ui, uo, f, faces, b, omega, march_xchunk(b), stream);
 }
 
+// LDS bytes the multi-sweep kernel needs for a level, 0 when it does not fit
+static inline size_t lds_sweeps_bytes(const mgp_array* u, size_t esize) {
+    const size_t b = ((size_t)(u->shape[0] + 2) * (u->shape[1] + 2) * (u->shape[2] + 2) +
+                      (size_t)u->shape[0] * u->shape[1] * u->shape[2]) * esize;
+    return b <= 65536 ? b : 0;
+}
+
+template <class T>
+static int rbgs_sweeps(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                       const int64_t b[6], double omega, int64_t nsweeps, void* stream) {
+    if (!valid_array(ui) || !valid_array(uo) || !valid_array(f) || !faces || !same_shape(ui, uo) ||
+        !same_shape(ui, f) || nsweeps < 0 || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    if (!checkerboard(mode, ui) || lin0_of(ui) != lin0_of(uo) || ui->stride[0] != uo->stride[0] ||
+        ui->stride[1] != uo->stride[1])
+        return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < 0 || b[2 * d + 1] > ui->shape[d]) return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == ui->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == ui->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == ui->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
+    const size_t lds = lds_sweeps_bytes(ui, sizeof(T));
+    if (!lds) return MGP_ERR_ARG;  // too large for one workgroup
+    if ((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) > 8 * 1024) return MGP_ERR_ARG;  // 8 points per thread
+    if (box_empty(bx) || nsweeps == 0) {
+        if (ui->data != uo->data && !box_empty(bx)) return MGP_ERR_ARG;
+        return MGP_OK;
+    }
+    const RelaxConsts<T> c(omega, faces->h);
+    const Faces<T> F = faces_of<T>(*faces);
+    const View<T> vi = view_of<T>(*ui);
+    const hipStream_t s = (hipStream_t)stream;
+    if (mode == MGP_MODE_REFERENCE)
+        hipLaunchKernelGGL((k_rbgs_lds<T, MGP_MODE_REFERENCE>), dim3(1), dim3(1024), lds, s, vi, view_of<T>(*uo),
+                           view_of<T>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, lin0_of(ui));
+    else
+        hipLaunchKernelGGL((k_rbgs_lds<T, MGP_MODE_CONSISTENT>), dim3(1), dim3(1024), lds, s, vi, view_of<T>(*uo),
+                           view_of<T>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, lin0_of(ui));
+    return launch_status();
+}
+
 template <class T>
 static int jacobi(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6],
                   double omega, double h, void* stream) {
@@ -720,6 +863,14 @@ int mgp_rbgs_sweep_f64(int m, const mgp_array* ui, const mgp_array* uo, const mg
 int mgp_rbgs_sweep_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                        const int64_t b[6], double w, void* s) {
     return rbgs_sweep<float>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweeps_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                        const int64_t b[6], double w, int64_t n, void* s) {
+    return rbgs_sweeps<double>(m, ui, uo, f, F, b, w, n, s);
+}
+int mgp_rbgs_sweeps_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                        const int64_t b[6], double w, int64_t n, void* s) {
+    return rbgs_sweeps<float>(m, ui, uo, f, F, b, w, n, s);
 }
 int mgp_jacobi_f64(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], double w,
                    double h, void* s) {

@@ -892,7 +892,8 @@ template int march_residual_to<double, float>(int, const mgp_array*, const mgp_a
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
 int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch
-int64_t g_sweep_min_chunk = 48;     // shortest x march per workgroup
+int64_t g_sweep_min_chunk = 4;      // shortest x march per workgroup / wave: coarse levels are
+                                    // latency-bound, so they get many short marches
 int64_t g_sweep_waves = 8;          // waves per workgroup: 4, 8 or 16
 int64_t g_sweep_kernel = 2;         // 1: workgroup tiles with LDS, 2: wave-private tiles
 int64_t g_sweep_rows = 0;           // register rows per wave tile: 8, 12, 16; 0: by type

@@ -20,6 +20,7 @@ INJECTION, FULL_WEIGHTING = 0, 1
 JACOBI, GAUSS_SEIDEL = 0, 1
 F64, F32, MIXED = 0, 1, 2
 NO_GRAPHS, NO_FUSED, QUIET = 1, 2, 4
+OK, ERR_ARG, ERR_NODEV, ERR_STATE = 0, -1, -2, -3
 LOC = "NSEWTB"
 
 
@@ -111,6 +112,7 @@ for sfx in ("f64", "f32"):
         "mgp_norm_inf_" + sfx: [P(Array), vp, vp],
         "mgp_rbgs_half_" + sfx: [i32, P(Array), P(Array), BOX, d, d, i32, vp],
         "mgp_rbgs_sweep_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, vp],
+        "mgp_rbgs_sweeps_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
         "mgp_jacobi_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, d, vp],
         "mgp_residual_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, vp, vp],
         "mgp_restrict_inject_" + sfx: [P(Array), P(Array), vp], "mgp_restrict_fw_" + sfx: [P(Array), P(Array), vp],

@@ -45,6 +45,18 @@ public:
         d.box(b);
         const mgp_faces F = d.faces();
         const mgp_array f = d.f->desc();
+        if (d.settings.fused_sweeps) {
+            // a level that fits one workgroup's LDS: every pass in one launch
+            // (the result lands where the last of nsmooth ping-pong passes would)
+            const mgp_array ui = d.u->desc();
+            mgp_array uo = ui;
+            if (nsmooth & 1) uo.data = d.u_alt;
+            if (mgpx::Ops<T>::rbgs_sweeps(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega, nsmooth,
+                                          mgpx::stream()) == MGP_OK) {
+                if (nsmooth & 1) d.swap_u_alt();
+                return;
+            }
+        }
         for (int_t s = 0; s < nsmooth; s++) {
             mgp_array ui = d.u->desc();
             mgp_array uo = ui;

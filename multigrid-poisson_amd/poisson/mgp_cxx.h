@@ -72,6 +72,10 @@ template <class T> struct Ops;
                               const mgp_faces* F, const int64_t b[6], double w, void* s) {                       \
             return mgp_rbgs_sweep_##SFX(m, ui, uo, f, F, b, w, s);                                                \
         }                                                                                                         \
+        static int rbgs_sweeps(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,              \
+                               const mgp_faces* F, const int64_t b[6], double w, int64_t n, void* s) {           \
+            return mgp_rbgs_sweeps_##SFX(m, ui, uo, f, F, b, w, n, s);                                            \
+        }                                                                                                         \
         static int jacobi(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], \
                           double w, double h, void* s) {                                                         \
             return mgp_jacobi_##SFX(m, u, v, f, b, w, h, s);                                                      \

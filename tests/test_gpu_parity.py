@@ -442,3 +442,70 @@ def test_wave_residual_equals_point_residual(shape, bc, mode, dtype):
     np.testing.assert_array_equal(out[1][0], out[2][0])
     np.testing.assert_array_equal(out[1][2], out[2][2])
     assert out[1][1] == out[2][1] and out[1][1] > 0
+
+
+LDS_SHAPES = [((17, 17, 9), "NNNNDD"), ((9, 9, 9), "DDDDDD"), ((9, 17, 5), "NDDNND"), ((5, 9, 17), "DNNDDN"),
+              ((3, 3, 3), "DDDDDD"), ((17, 9, 9), "NNNNNN")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("nsweeps", [1, 2, 5])
+@pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
+@pytest.mark.parametrize("shape,bc", LDS_SHAPES, ids=[str(i) for i in range(len(LDS_SHAPES))])
+def test_lds_multi_sweep_equals_single_sweeps(shape, bc, mode, nsweeps, dtype):
+    """mgp_rbgs_sweeps (every pass of a small level in one workgroup's LDS,
+    one launch) == nsweeps one-pass sweeps, bit for bit, in place and out of
+    place, on levels with Neumann and Dirichlet faces."""
+    say("case lds sweeps", shape, bc, mode, nsweeps, dtype.__name__)
+    halo = [1 if c == "N" else 0 for c in bc]
+    if mode == mgp.REFERENCE:
+        eff = ["D" if q in (0, 1, 5) else bc[q] for q in range(6)]  # north, south, bottom forced Dirichlet
+        p = mgp.padded_shape(list(shape), halo)
+        if not ((p[1] * p[2]) % 2 and p[2] % 2):
+            pytest.skip("reference colouring is not a checkerboard for this halo layout")
+    else:
+        eff = list(bc)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc, mode, nsweeps)).encode()))
+    sfx = "f64" if dtype == np.float64 else "f32"
+    u0 = rng.standard_normal(mgp.padded_shape(list(shape), halo))
+    f0 = rng.standard_normal(mgp.padded_shape(list(shape), halo))
+    f = mgp.DeviceArray(shape, halo, dtype=dtype).upload(f0)
+    faces = {}
+    F = mgp.Faces()
+    for q in range(6):
+        n = [shape[0], shape[1], shape[2]]
+        n[{0: 1, 1: 1, 2: 0, 3: 0, 4: 2, 5: 2}[q]] = 1
+        faces[q] = mgp.DeviceBuffer(rng.standard_normal(int(np.prod(n))).astype(dtype))
+        F.type[q] = mgp.NEUMANN if eff[q] == "N" else mgp.DIRICHLET
+        F.g[q] = faces[q].ptr if eff[q] == "N" else None
+    F.h = 1.0 / (shape[0] - 1)
+    box = [int(eff[3] == "D"), shape[0] - int(eff[2] == "D"), int(eff[1] == "D"), shape[1] - int(eff[0] == "D"),
+           int(eff[5] == "D"), shape[2] - int(eff[4] == "D")]
+    # nsweeps single sweeps, ping-pong
+    a = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    b = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    for _ in range(nsweeps):
+        mgp.call("mgp_rbgs_sweep_" + sfx, mode, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93, None)
+        a, b = b, a
+    want = a.download()
+    # one launch, out of place and in place
+    c = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    d = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    mgp.call("mgp_rbgs_sweeps_" + sfx, mode, c.ref(), d.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
+             nsweeps, None)
+    mgp.call("mgp_rbgs_sweeps_" + sfx, mode, c.ref(), c.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
+             nsweeps, None)
+    np.testing.assert_array_equal(d.download(), want)
+    np.testing.assert_array_equal(c.download(), want)
+
+
+def test_lds_multi_sweep_rejects_large_levels():
+    shape, halo = (65, 65, 33), (1, 1, 1, 1, 0, 0)
+    u = mgp.DeviceArray(shape, halo)
+    F = mgp.Faces()
+    for q in range(6):
+        F.type[q] = mgp.DIRICHLET
+    F.h = 1.0 / 64
+    rc = mgp.lib().mgp_rbgs_sweeps_f64(mgp.CONSISTENT, u.ref(), u.ref(), u.ref(), ctypes.byref(F),
+                                       mgp.box([1, 64, 1, 64, 1, 32]), 1.0, 3, None)
+    assert rc == mgp.ERR_ARG
