@@ -202,7 +202,8 @@ int mgp_event_elapsed_ms(void* start, void* stop, float* ms);
 enum { MGP_JACOBI = 0, MGP_GAUSS_SEIDEL = 1 };
 #define MGP_SOLVER_NO_GRAPHS 1 /* run cycles launch by launch (Settings::use_graphs = false) */
 #define MGP_SOLVER_NO_FUSED 2  /* red-black half-sweeps only (Settings::fused_sweeps = false) */
-#define MGP_SOLVER_QUIET 4     /* no per-cycle timing line (Settings::print_cycle_time = false) */
+#define MGP_SOLVER_QUIET 4     /* no per-cycle timing line, no level listing at creation
+                                  (Settings::print_cycle_time, print_settings = false) */
 
 typedef struct mgp_solver_config {
     int64_t dims[3];     /* Settings::dims (include/settings.h:17) */
@@ -220,6 +221,8 @@ typedef struct mgp_solver_config {
     int64_t miniter;     /* Settings::miniter */
     double tolerance;    /* Settings::tolerance */
     int64_t coarse_sweeps; /* Settings::coarse_sweeps (consistent mode; 0 = nsmooth) */
+    int64_t refine_polish; /* MGP_MIXED: FP64 smoothing passes on the finest level after each
+                              update (Settings::refine_polish; 0 = none) */
 } mgp_solver_config;
 
 /* PoissonSolver(settings, BC) (libpoisson.h:79-87) */
@@ -261,6 +264,17 @@ int mgp_solver_face(void* solver, int level, int loc, void** host, mgp_array* de
 int mgp_solver_save(void* solver, const char* path);
 /* message of the last failing mgp_solver_* / mgp_batch_* call on this thread */
 const char* mgp_last_error(void);
+
+/* Consistent mode: coarse f = full weighting (include/fullweighting.h:59-121)
+ * of r = f - L u (include/residual.h:58-79), r computed on the box and not
+ * stored, r outside the box read as stored; u's halos must hold current
+ * ghosts.  The descent step of a cycle (residual, then restriction) in one
+ * pass over the fine level.  MGP_ERR_ARG in reference mode or when the
+ * layouts do not allow it. */
+int mgp_residual_restrict_fw_f64(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                                 const mgp_array* coarse_f, const int64_t box[6], double h, void* stream);
+int mgp_residual_restrict_fw_f32(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                                 const mgp_array* coarse_f, const int64_t box[6], double h, void* stream);
 
 /* residual of FP64 u and f on the box, written as FP32 into `out` (same
  * shape, halos and element strides as u) instead of r; with norm, max |r|
@@ -377,7 +391,9 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
  * loads in flight per wave, kernel 2, default 1), "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
- * "residual.rows" (8 or 16 register rows per residual tile, default 16).
+ * "residual.rows" (8 or 16 register rows per residual tile, default 16),
+ * "prolong.kernel" (1: per-point kernel; 2: wave tiles marching along x for
+ * mgp_prolong_add_*, default).
  * No counterpart in the
  * reference, whose launch
  * shape was fixed by the OpenMP runtime. */

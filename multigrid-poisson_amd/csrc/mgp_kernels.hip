@@ -695,6 +695,7 @@ static int prolong(const mgp_array* coarse, const mgp_array* fine, void* stream)
     if (!valid_array(coarse) || !valid_array(fine)) return MGP_ERR_ARG;
     for (int d = 0; d < 3; d++)  // exact 2:1 nesting (trilinearinterpolation.h:31-37)
         if (2 * (coarse->shape[d] - 1) + 1 != fine->shape[d]) return MGP_ERR_ARG;
+    if (ADD && g_prolong_kernel == 2 && march_prolong_add<T>(coarse, fine, stream) == MGP_OK) return MGP_OK;
     const Box bx = shape_box(fine);
     const BoxGrid g = box_grid(bx);
     hipLaunchKernelGGL((k_prolong<T, ADD>), g.grid, g.block, 0, (hipStream_t)stream, view_of<T>(*coarse),
@@ -774,6 +775,15 @@ static int bc_restrict_neumann(int restriction, const mgp_array* uf, int loc, co
     return launch_status();
 }
 
+template <class T>
+static int residual_restrict_fw(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                                const mgp_array* cf, const int64_t b[6], double h, void* stream) {
+    if (mode != MGP_MODE_CONSISTENT || !valid_array(u) || !valid_array(f) || !valid_array(r) || !valid_array(cf) ||
+        !same_shape(u, f) || !same_shape(u, r) || !coarse_of(u, cf) || !valid_box(u, b))
+        return MGP_ERR_ARG;
+    return march_residual_fw<T>(u, f, r, cf, b, h, stream);
+}
+
 }  // namespace mgp
 
 // ============================================================ C entry points
@@ -830,6 +840,15 @@ int mgp_fill_modes_f64(const mgp_array* f, int64_t nmodes, const int32_t* abc, c
     hipLaunchKernelGGL(k_fill_modes<double>, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*f), bx, g.zchunks,
                        m, h);
     return launch_status();
+}
+
+int mgp_residual_restrict_fw_f64(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                                 const mgp_array* cf, const int64_t b[6], double h, void* s) {
+    return residual_restrict_fw<double>(m, u, f, r, cf, b, h, s);
+}
+int mgp_residual_restrict_fw_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,
+                                 const mgp_array* cf, const int64_t b[6], double h, void* s) {
+    return residual_restrict_fw<float>(m, u, f, r, cf, b, h, s);
 }
 
 int mgp_residual_cast_f64_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,

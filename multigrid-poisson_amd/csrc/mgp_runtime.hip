@@ -127,6 +127,7 @@ int mgp_tuning_set(const char* key, int64_t value) {
     else if (!strcmp(key, "sweep.depth") && (value == 1 || value == 2)) mgp::g_sweep_depth = value;
     else if (!strcmp(key, "residual.kernel") && (value == 1 || value == 2)) mgp::g_resid_kernel = value;
     else if (!strcmp(key, "residual.rows") && (value == 8 || value == 16)) mgp::g_resid_rows = value;
+    else if (!strcmp(key, "prolong.kernel") && (value == 1 || value == 2)) mgp::g_prolong_kernel = value;
     else return MGP_ERR_ARG;
     return MGP_OK;
 }
@@ -141,6 +142,7 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "sweep.depth")) *value = mgp::g_sweep_depth;
     else if (!strcmp(key, "residual.kernel")) *value = mgp::g_resid_kernel;
     else if (!strcmp(key, "residual.rows")) *value = mgp::g_resid_rows;
+    else if (!strcmp(key, "prolong.kernel")) *value = mgp::g_prolong_kernel;
     else return MGP_ERR_ARG;
     return MGP_OK;
 }

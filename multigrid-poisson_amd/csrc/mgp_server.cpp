@@ -2,19 +2,20 @@
 // types accepted as they arrive and run concurrently on one GPU.  No
 // counterpart in the reference, which runs one solve per process.
 //
-// A worker thread owns the device work.  Each admitted request gets a slot: a
-// RefinementSolver (FP32 F-cycles inside FP64 iterative refinement) with its
-// own stream.  Slots stay resident after their request ends and are reused by
-// later requests of the same shape and boundary types; an HBM budget bounds
-// active plus resident slots, and idle slots are evicted least recently used
-// first when a new shape needs room.  The worker admits pending requests in
-// arrival order while a stream slot and memory are free, enqueues one
-// refinement iteration per active solve at a time, and polls each solve's
-// event; so iterations of different requests overlap on the device and a
-// request that arrives mid-way joins at the next poll.  A request's arithmetic
-// does not depend on what else runs (each solve has its own arrays and
-// stream, every kernel is deterministic), so its result is the same bits
-// alone or in a mix.
+// max_active worker threads each own one stream slot at a time.  A worker
+// takes the oldest pending request once the HBM budget allows its hierarchy
+// (a RefinementSolver: FP32 F-cycles inside FP64 iterative refinement, with
+// its own stream), solves it with the same solve() a lone caller would run,
+// copies the solution to the caller's buffer and takes the next request.
+// Hierarchies stay resident after their request ends and are reused by later
+// requests of the same shape and face types; idle ones are evicted least
+// recently used first when a new shape needs room.  Requests that arrive
+// while others run are admitted as workers and memory free up; their kernels
+// overlap on the device (one stream each) and their host work (hierarchy
+// set-up, graph recording, result copies) overlaps across worker threads.  A
+// request's arithmetic does not depend on what else runs (own arrays and
+// stream, deterministic kernels), so its result is the same bits alone or in
+// a mix.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -75,14 +76,10 @@ struct SlotBase {
     Key key;
     size_t bytes = 0;
     Job* job = nullptr;
-    hipEvent_t ev = nullptr;
     std::vector<double> staging;  // host copy of the padded solution
-    virtual ~SlotBase() {
-        if (ev) (void)hipEventDestroy(ev);
-    }
-    virtual void begin(const mgp_request& r) = 0;  // f, u = 0, ||f||, initial residual (waits)
-    virtual void enqueue() = 0;                     // one iteration + its norm, then the event
-    virtual bool step_done() = 0;                   // stop rule after the event
+    virtual ~SlotBase() {}
+    virtual void begin(const mgp_request& r) = 0;  // f, u = 0 (waits)
+    virtual void run() = 0;                         // the solve (waits)
     virtual void finish(double* out) = 0;           // solution to the host (waits)
     virtual void stats(int64_t* iters, double* rel) = 0;
 };
@@ -109,26 +106,21 @@ struct Slot : SlotBase {
         st.coarse_sweeps = c.coarse_sweeps;
         st.mode = Poisson::CONSISTENT;
         st.print_cycle_time = false;
+        st.print_settings = false;
         auto t = [&](int q) { return r.bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
         bc = Poisson::BoundaryCondition{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
         solver.reset(new Poisson::RefinementSolver<Poisson::FullWeighting, Poisson::TrilinearInterpolation, S>(st, bc));
         solver->to_device_zero();  // zero f, u and (zero) face data, on the device
         nsmooth = c.nsmooth;
         omega = c.omega;
-        hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "server event");
     }
     void begin(const mgp_request& r) override {
         Poisson::Domain<double>& d = solver->fp64().get_domain(0, 0);
         const mgp_array f = d.f->desc();
         Poisson::mgpx::check(mgp_fill_modes_f64(&f, r.nmodes, r.abc, r.amp, st.h, solver->get_stream()), "rhs");
         solver->reset_solution();  // u = 0 with the (zero) boundary data; waits
-        solver->start("Fcycle");
     }
-    void enqueue() override {
-        solver->enqueue_iteration(nsmooth, omega);
-        hip_check(hipEventRecord(ev, (hipStream_t)solver->get_stream()), "server event");
-    }
-    bool step_done() override { return solver->iteration_done(); }
+    void run() override { solver->solve("Fcycle", nsmooth, omega, "", 1e30); }
     void finish(double* out) override {
         const mgp_array u = solver->fp64().get_domain(0, 0).u->desc();
         const int64_t p0 = u.shape[0] + u.halo[MGP_EAST] + u.halo[MGP_WEST];
@@ -157,7 +149,9 @@ public:
         if (c.hbm_budget <= 0 || c.max_active < 1 || c.nsmooth < 1 || c.maxiter < 1 || !(c.tolerance >= 0.0) ||
             (c.relaxation != MGP_GAUSS_SEIDEL && c.relaxation != MGP_JACOBI))
             throw std::invalid_argument("invalid server config");
-        worker = std::thread([this] { run(); });
+        int dev = 0;
+        hip_check(hipGetDevice(&dev), "server");
+        for (int w = 0; w < c.max_active; w++) workers.emplace_back([this, dev] { run(dev); });
     }
     ~Server() {
         {
@@ -165,7 +159,8 @@ public:
             stopping = true;
         }
         cv_work.notify_all();
-        worker.join();
+        for (auto& t : workers) t.join();
+        idle.clear();
     }
 
     int64_t submit(const mgp_request& r) {
@@ -213,77 +208,57 @@ public:
 
 private:
     mgp_server_config cfg;
-    std::thread worker;
-    std::mutex mu;
+    std::vector<std::thread> workers;
+    std::mutex mu;  // guards everything below
     std::condition_variable cv_work, cv_done;
     bool stopping = false;
     int64_t next_id = 1;
     std::deque<Job*> pending;
     std::map<int64_t, std::unique_ptr<Job>> jobs;
-    // worker-owned
-    std::vector<std::unique_ptr<SlotBase>> active;
-    std::list<std::unique_ptr<SlotBase>> idle;  // least recently used first
-    size_t bytes_used = 0;
-    int64_t completed = 0, peak_active = 0;
+    std::list<std::unique_ptr<SlotBase>> idle;  // resident, least recently used first
+    size_t bytes_used = 0;                      // active + resident hierarchies (and reservations)
+    int64_t active = 0, completed = 0, peak_active = 0;
 
     static Key key_of(const mgp_request& r) {
         uint32_t bits = 0;
         for (int q = 0; q < 6; q++) bits |= (uint32_t)(r.bc[q] == MGP_NEUMANN) << q;
         return Key{r.dims[0], r.dims[1], r.dims[2], bits};
     }
-
-    std::unique_ptr<SlotBase> make_slot(const mgp_request& r) {
-        size_t free0 = 0, total = 0, free1 = 0;
-        hip_check(hipMemGetInfo(&free0, &total), "server");
-        std::unique_ptr<SlotBase> s;
-        if (cfg.relaxation == MGP_GAUSS_SEIDEL) s.reset(new Slot<Poisson::GaussSeidel>(r, cfg));
-        else s.reset(new Slot<Poisson::Jacobi>(r, cfg));
-        hip_check(hipMemGetInfo(&free1, &total), "server");
-        s->key = key_of(r);
-        s->bytes = free0 > free1 ? free0 - free1 : 0;
-        return s;
+    // device bytes of a request's hierarchy: level 0 in FP64 (u, u_alt, f, r)
+    // and the FP32 hierarchy (u, u_alt, f, r; 8/7 of level 0), halos included
+    static size_t need_of(const mgp_request& r) {
+        const double pts = (double)(r.dims[0] + 2) * (double)(r.dims[1] + 2) * (double)(r.dims[2] + 2);
+        return (size_t)(pts * (4 * 8 + 4 * 4 * 8.0 / 7.0) * 1.05) + (4u << 20);
     }
 
-    // a resident slot for the request, or nullptr when memory does not allow one now
-    std::unique_ptr<SlotBase> acquire(const mgp_request& r) {
-        const Key k = key_of(r);
+    // under mu: the oldest request and a resident slot for it, or a
+    // reservation of its bytes for a new slot; false when nothing can start
+    bool take(Job*& j, std::unique_ptr<SlotBase>& s, size_t& reserved) {
+        if (pending.empty()) return false;
+        j = pending.front();
+        const Key k = key_of(j->req);
         for (auto it = idle.begin(); it != idle.end(); ++it)
             if ((*it)->key == k) {
-                std::unique_ptr<SlotBase> s = std::move(*it);
+                s = std::move(*it);
                 idle.erase(it);
-                return s;
+                pending.pop_front();
+                reserved = 0;
+                return true;
             }
-        // estimate: level 0 in FP64 (u, u_alt, f, r) plus the FP32 hierarchy
-        // (u, u_alt, f, r; 8/7 of level 0) plus padding
-        const double pts = (double)(r.dims[0] + 2) * (double)(r.dims[1] + 2) * (double)(r.dims[2] + 2);
-        const size_t need = (size_t)(pts * (4 * 8 + 4 * 4 * 8.0 / 7.0) * 1.05) + (8u << 20);
+        const size_t need = need_of(j->req);
         while (bytes_used + need > (size_t)cfg.hbm_budget && !idle.empty()) {
             bytes_used -= idle.front()->bytes;
             idle.pop_front();  // frees its device arrays
         }
-        if (bytes_used + need > (size_t)cfg.hbm_budget && !active.empty()) return nullptr;  // wait for room
-        std::unique_ptr<SlotBase> s = make_slot(r);
-        bytes_used += s->bytes;
-        return s;
+        // wait for room unless nothing else holds memory (then try anyway)
+        if (bytes_used + need > (size_t)cfg.hbm_budget && active > 0) return false;
+        bytes_used += need;
+        reserved = need;
+        pending.pop_front();
+        return true;
     }
 
-    void complete(SlotBase& s, int status) {
-        Job* j = s.job;
-        if (status == MGP_OK) {
-            try {
-                const auto t0 = Clock::now();
-                s.finish(j->req.u_out);
-                j->res.fetch_s = seconds(t0, Clock::now());
-                s.stats(&j->res.iterations, &j->res.rel_res);
-            } catch (const std::exception& e) {
-                mgp::set_last_error(e.what());
-                status = MGP_ERR_STATE;
-            }
-        }
-        s.job = nullptr;
-        complete_job(j, status);
-    }
-    void complete_job(Job* j, int status) {
+    void finish_job(Job* j, int status) {
         j->t_done = Clock::now();
         std::lock_guard<std::mutex> lk(mu);
         j->res.status = status;
@@ -295,90 +270,58 @@ private:
         cv_done.notify_all();
     }
 
-    void run() {
+    void run(int dev) {
+        (void)hipSetDevice(dev);
         while (true) {
-            // admission, in arrival order
-            while ((int)active.size() < cfg.max_active) {
-                Job* j = nullptr;
-                {
+            Job* j = nullptr;
+            std::unique_ptr<SlotBase> s;
+            size_t reserved = 0;
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv_work.wait(lk, [&] { return (stopping && pending.empty()) || take(j, s, reserved); });
+                if (!j) return;  // stopping, nothing pending
+                active++;
+                peak_active = std::max(peak_active, active);
+            }
+            j->t_admit = Clock::now();
+            int status = MGP_OK;
+            try {
+                if (!s) {
+                    s = make_slot(j->req);
                     std::lock_guard<std::mutex> lk(mu);
-                    if (pending.empty()) break;
-                    j = pending.front();
+                    // replace the estimate by what the allocation took
+                    bytes_used = bytes_used - reserved + s->bytes;
+                    reserved = 0;
                 }
-                std::unique_ptr<SlotBase> s;
-                j->t_admit = Clock::now();
-                try {
-                    s = acquire(j->req);
-                } catch (const std::exception& e) {
-                    // e.g. out of device memory with nothing else running
-                    mgp::set_last_error(e.what());
-                    {
-                        std::lock_guard<std::mutex> lk(mu);
-                        pending.pop_front();
-                    }
-                    complete_job(j, MGP_ERR_STATE);
-                    continue;
-                }
-                if (!s) break;  // no memory until an active solve ends
-                {
-                    std::lock_guard<std::mutex> lk(mu);
-                    pending.pop_front();
-                }
-                s->job = j;
-                try {
-                    s->begin(j->req);
-                    j->res.setup_s = seconds(j->t_admit, Clock::now());
-                    s->enqueue();
-                } catch (const std::exception& e) {
-                    mgp::set_last_error(e.what());
-                    complete(*s, MGP_ERR_STATE);
-                    idle.push_back(std::move(s));
-                    continue;
-                }
-                active.push_back(std::move(s));
+                s->begin(j->req);
+                j->res.setup_s = seconds(j->t_admit, Clock::now());
+                s->run();
+                const auto t0 = Clock::now();
+                s->finish(j->req.u_out);
+                j->res.fetch_s = seconds(t0, Clock::now());
+                s->stats(&j->res.iterations, &j->res.rel_res);
+            } catch (const std::exception& e) {
+                mgp::set_last_error(e.what());
+                status = MGP_ERR_STATE;
+            }
+            {
                 std::lock_guard<std::mutex> lk(mu);
-                peak_active = std::max<int64_t>(peak_active, (int64_t)active.size());
+                bytes_used -= reserved;
+                if (s) idle.push_back(std::move(s));
+                active--;
             }
-            // progress of the active solves
-            bool progressed = false;
-            for (size_t a = 0; a < active.size();) {
-                SlotBase& s = *active[a];
-                const hipError_t q = hipEventQuery(s.ev);
-                if (q == hipErrorNotReady) {
-                    a++;
-                    continue;
-                }
-                progressed = true;
-                bool end = true;
-                int status = MGP_OK;
-                try {
-                    hip_check(q, "server iteration");
-                    end = s.step_done();
-                    if (!end) s.enqueue();
-                } catch (const std::exception& e) {
-                    mgp::set_last_error(e.what());
-                    status = MGP_ERR_STATE;
-                    end = true;
-                }
-                if (!end) {
-                    a++;
-                    continue;
-                }
-                complete(s, status);
-                idle.push_back(std::move(active[a]));
-                active.erase(active.begin() + a);
-            }
-            std::unique_lock<std::mutex> lk(mu);
-            if (stopping && pending.empty() && active.empty()) break;
-            if (active.empty() && pending.empty()) {
-                cv_work.wait(lk, [&] { return stopping || !pending.empty(); });
-            } else if (!progressed) {
-                lk.unlock();
-                std::this_thread::sleep_for(std::chrono::microseconds(20));
-            }
+            cv_work.notify_all();  // memory or a worker came free
+            finish_job(j, status);
         }
-        active.clear();
-        idle.clear();
+    }
+
+    std::unique_ptr<SlotBase> make_slot(const mgp_request& r) {
+        std::unique_ptr<SlotBase> s;
+        if (cfg.relaxation == MGP_GAUSS_SEIDEL) s.reset(new Slot<Poisson::GaussSeidel>(r, cfg));
+        else s.reset(new Slot<Poisson::Jacobi>(r, cfg));
+        s->key = key_of(r);
+        s->bytes = need_of(r);
+        return s;
     }
 };
 

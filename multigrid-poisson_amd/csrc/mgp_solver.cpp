@@ -200,6 +200,9 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
         st.use_graphs = !(c->flags & MGP_SOLVER_NO_GRAPHS);
         st.fused_sweeps = !(c->flags & MGP_SOLVER_NO_FUSED);
         st.print_cycle_time = !(c->flags & MGP_SOLVER_QUIET);
+        st.print_settings = !(c->flags & MGP_SOLVER_QUIET);
+        if (c->refine_polish < 0) throw std::invalid_argument("refine_polish must be >= 0");
+        st.refine_polish = c->refine_polish;
         auto t = [&](int q) { return c->bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
         Poisson::BoundaryCondition bc{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
         if (c->dtype == MGP_MIXED) {

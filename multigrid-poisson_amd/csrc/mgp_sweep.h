@@ -20,6 +20,15 @@ int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_a
 template <class T, class O>
 int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
                       const int64_t box[6], double h, double* norm, void* stream);
+// u += trilinear P(coarse) over the fine shape through wave tiles
+template <class T>
+int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stream);
+extern int64_t g_prolong_kernel;
+// coarse f = full weighting of (f - L u) on the fine box in one pass
+// (consistent mode; r outside the box read as stored, r itself not written)
+template <class T>
+int march_residual_fw(const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* cf,
+                      const int64_t box[6], double h, void* stream);
 extern int64_t g_resid_kernel;
 extern int64_t g_resid_rows;
 // default x-chunk length for a box

@@ -730,6 +730,289 @@ __global__ void k_norm_region(View<T> r, int64_t x0, int64_t x1, int64_t y0, int
     if ((threadIdx.x & 63) == 0 && v > 0.0) atomicMax(norm, (unsigned long long)__double_as_longlong(v));
 }
 
+
+// ---------------------------------------------------------------------------
+// Trilinear prolongation added to the fine solution, u += P(coarse), over the
+// fine shape (include/trilinearinterpolation.h:50-133 composed as the four
+// passes compose it, then include/devicearray.h:175-194).  One wave per fine
+// (y, z) tile of R rows x 64 columns marching along x: the fine plane is read,
+// updated and written; the coarse values it needs (coarse rows b and b+1,
+// columns c and c+1, planes a and a+1) are loaded into registers once per
+// coarse plane and shared by the two fine planes that use them.
+template <class T>
+struct ProlongArgs {
+    View<T> c, u;  // coarse, fine
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t xchunk;  // even
+};
+
+template <class T, int R>
+__global__ void __launch_bounds__(64) k_prolong_add_wave(ProlongArgs<T> A) {
+    static_assert(R % 2 == 0, "fine tiles start on even rows");
+    constexpr int CR = R / 2 + 1;  // coarse rows a tile reads
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2;
+    const int64_t c0 = A.c.n0, c1 = A.c.n1, c2 = A.c.n2;
+    const int64_t y0 = (int64_t)ty * R, z0 = (int64_t)tz * 64;
+    const int64_t xa = (int64_t)xc * A.xchunk, xb = min(n0, xa + A.xchunk);
+    const int lane = threadIdx.x;
+    const int64_t k = z0 + lane;
+    const bool lane_in = k < n2;
+    const int64_t kc = min(k, n2 - 1);
+    const int pk = (int)(kc & 1);
+    const uint32_t zoff = (uint32_t)(kc * (int64_t)sizeof(T));
+    const uint32_t zoff_st = lane_in ? zoff : BUF_OOB;
+    // coarse columns c = k>>1 and c+1 (clamped; c+1 is only used for odd k)
+    const uint32_t czo = (uint32_t)(min(kc >> 1, c2 - 1) * (int64_t)sizeof(T));
+    const uint32_t czo1 = (uint32_t)(min((kc >> 1) + 1, c2 - 1) * (int64_t)sizeof(T));
+    const uint32_t fplane = (uint32_t)(((n1 - 1) * A.u.s1 + n2) * (int64_t)sizeof(T));
+    const uint32_t cplane = (uint32_t)(((c1 - 1) * A.c.s1 + c2) * (int64_t)sizeof(T));
+    uint32_t froff[R], croff[CR];
+    bool row_in[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int64_t j = y0 + t;
+        row_in[t] = j < n1;
+        froff[t] = (uint32_t)(min(j, n1 - 1) * A.u.s1 * (int64_t)sizeof(T));
+    }
+#pragma unroll
+    for (int t = 0; t < CR; t++) croff[t] = (uint32_t)(min((y0 >> 1) + t, c1 - 1) * A.c.s1 * (int64_t)sizeof(T));
+    // coarse plane a: values at (row b, column c) and (row b, column c+1)
+    auto load_coarse = [&](int64_t a, T (&V)[CR], T (&V1)[CR]) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.c.p + min(a, c0 - 1) * A.c.s0, cplane);
+#pragma unroll
+        for (int t = 0; t < CR; t++) {
+            V[t] = bload<T>(r, czo, croff[t]);
+            V1[t] = bload<T>(r, czo1, croff[t]);
+        }
+    };
+    auto half_sum = [](T a, T b) -> T { return 0.5 * (a + b); };  // trilinearinterpolation.h: 0.5 * (l + r)
+    // xy value at coarse column set (V: plane a, W: plane a+1) for fine row t
+    // of parity pj, fine plane parity pi (the reference's x-then-y order)
+    auto tri_xy = [&](const T (&V)[CR], const T (&W)[CR], int t, int pi) -> T {
+        const int b = t >> 1, pj = t & 1;  // y0 is even
+        if (!pi && !pj) return V[b];
+        if (pi && !pj) return half_sum(V[b], W[b]);
+        if (!pi && pj) return half_sum(V[b], V[b + 1]);
+        return half_sum(half_sum(V[b], W[b]), half_sum(V[b + 1], W[b + 1]));
+    };
+    T Ca[CR], Ca1[CR], Cb[CR], Cb1[CR], U[R], UL[R];
+    load_coarse(xa >> 1, Ca, Ca1);
+    load_coarse((xa >> 1) + 1, Cb, Cb1);
+    auto load_fine = [&](int64_t i, T (&D)[R]) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.u.p + min(i, n0 - 1) * A.u.s0, fplane);
+#pragma unroll
+        for (int t = 0; t < R; t++) D[t] = bload<T>(r, zoff, froff[t]);
+    };
+    load_fine(xa, U);
+    for (int64_t i = xa; i < xb; i++) {
+        load_fine(i + 1, UL);
+        const int pi = (int)(i & 1);
+        const __amdgpu_buffer_rsrc_t ro = plane_rsrc(A.u.p + i * A.u.s0, fplane);
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            const T lo = tri_xy(Ca, Cb, t, pi);
+            T v = lo;
+            if (pk) v = half_sum(lo, tri_xy(Ca1, Cb1, t, pi));
+            bstore((T)(U[t] + v), ro, row_in[t] ? zoff_st : BUF_OOB, froff[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < R; t++) U[t] = UL[t];
+        if (pi) {  // the next fine plane is even: coarse plane a+1 becomes a
+#pragma unroll
+            for (int t = 0; t < CR; t++) Ca[t] = Cb[t], Ca1[t] = Cb1[t];
+            load_coarse(((i + 1) >> 1) + 1, Cb, Cb1);
+        }
+    }
+}
+
+
+// ---------------------------------------------------------------------------
+// Residual and full-weighting restriction in one pass (consistent mode):
+// coarse f = FW(r) over the coarse shape with r = f - L u computed on the fine
+// box in registers and never stored (the fine r is dead after the restriction
+// when the prolongation is fused), r outside the box read as stored, and the
+// neighbour indices clamped to the shape (include/fullweighting.h:59-121,
+// include/residual.h:58-79).  One wave per coarse tile of RJ rows x 30
+// columns marching along x: fine u rows 2J0-2 .. 2J0+2RJ (NU), r rows
+// 2J0-1 .. 2J0+2RJ-1 (NRr), fine lanes z = 2K0-2+l; coarse column K sits on
+// lane 2(K-K0)+2.  Coarse plane I is emitted after r of fine plane 2I+1.
+template <class T>
+struct RFArgs {
+    View<T> u, f, r, c;  // fine u, f, r; coarse f
+    int64_t bx0, bx1, by0, by1, bz0, bz1;
+    int64_t ylo, yhi, zlo, zhi, xlo, xhi;  // allocated index ranges of the fine arrays
+    T hsq, inv_h2;
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t ichunk;  // coarse planes per wave
+};
+
+constexpr int FWK = 30;  // coarse columns per wave
+
+template <class T, int RJ, bool EDGE>
+__device__ __forceinline__ void resid_fw_march(const RFArgs<T>& A, int tl) {
+    constexpr int NU = 2 * RJ + 3, NRr = 2 * RJ + 1;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2;
+    const int64_t c0 = A.c.n0, c1 = A.c.n1, c2 = A.c.n2;
+    const int64_t J0 = (int64_t)ty * RJ, K0 = (int64_t)tz * FWK;
+    const int64_t Ia = (int64_t)xc * A.ichunk, Ib = min(c0, Ia + A.ichunk);
+    const int64_t s0 = A.u.s0, s1 = A.u.s1;
+    const uint32_t plane_bytes = (uint32_t)(((A.yhi - A.ylo) * s1 + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(T));
+    const int64_t corner = A.ylo * s1 + A.zlo;
+    const int lane = threadIdx.x;
+    const int64_t z = 2 * K0 - 2 + lane;
+    const int64_t zc = min(max(z, A.zlo), A.zhi);
+    const uint32_t zoff = (uint32_t)((zc - A.zlo) * (int64_t)sizeof(T));
+    const bool z_box = z >= A.bz0 && z < A.bz1 && lane >= 1 && lane <= 62;
+    const bool z_shape = z >= 0 && z < n2;
+    uint32_t roff[NU];
+    bool y_box[NU], y_shape[NU];
+#pragma unroll
+    for (int t = 0; t < NU; t++) {
+        const int64_t y = 2 * J0 - 2 + t;
+        roff[t] = (uint32_t)((min(max(y, A.ylo), A.yhi) - A.ylo) * s1 * (int64_t)sizeof(T));
+        y_box[t] = y >= A.by0 && y < A.by1;
+        y_shape[t] = y >= 0 && y < n1;
+    }
+    auto rsrc = [&](const T* p, int64_t i) {
+        return plane_rsrc(p + min(max(i, A.xlo), A.xhi) * s0 + corner, plane_bytes);
+    };
+    auto load_u = [&](int64_t i, T (&U)[NU]) {
+        const __amdgpu_buffer_rsrc_t rs = rsrc(A.u.p, i);
+#pragma unroll
+        for (int t = 0; t < NU; t++) U[t] = bload<T>(rs, zoff, roff[t]);
+    };
+    auto load_f = [&](int64_t i, T (&F)[NU]) {
+        const __amdgpu_buffer_rsrc_t rs = rsrc(A.f.p, i);
+#pragma unroll
+        for (int t = 1; t <= NRr; t++) F[t] = bload<T>(rs, zoff, roff[t]);
+    };
+    // r of fine plane i at rows t = 1..NRr (r row t-1): computed on the box,
+    // as stored elsewhere in the shape; outside the shape the clamped
+    // neighbour's value (rows and lanes; planes are handled at emission)
+    auto r_plane = [&](int64_t i, const T (&Um)[NU], const T (&U0)[NU], const T (&Up)[NU], const T (&F0)[NU],
+                       T (&Rr)[NRr]) {
+        const bool x_box = i >= A.bx0 && i < A.bx1;
+        __amdgpu_buffer_rsrc_t rs;
+        if (EDGE) rs = rsrc(A.r.p, i);
+#pragma unroll
+        for (int t = 1; t <= NRr; t++) {
+            const T v = resid_point<T, MGP_MODE_CONSISTENT>(U0[t], Um[t], Up[t], U0[t - 1], U0[t + 1],
+                                                            lane_prev(U0[t]), lane_next(U0[t]), F0[t], A.hsq,
+                                                            A.inv_h2);
+            if (!EDGE) {
+                Rr[t - 1] = v;
+            } else {
+                const bool in = x_box && y_box[t] && z_box;
+                const T st = bload<T>(rs, zoff, roff[t]);  // stored r (in range: clamped offsets)
+                Rr[t - 1] = in ? v : st;
+            }
+        }
+        if (EDGE) {
+            // rows outside the shape take the clamped row's value (y = -1 -> 0, y = n1 -> n1-1)
+#pragma unroll
+            for (int t = 1; t <= NRr; t++) {
+                const int64_t y = 2 * J0 - 2 + t;
+                if (y < 0 && t < NRr) Rr[t - 1] = Rr[t];
+                if (y >= n1 && t > 1) Rr[t - 1] = Rr[t - 2];
+            }
+            // lanes outside the shape: z = -1 -> lane+1, z = n2 -> lane-1
+#pragma unroll
+            for (int t = 0; t < NRr; t++) {
+                const T nx = lane_next(Rr[t]), pv = lane_prev(Rr[t]);
+                Rr[t] = z == -1 ? nx : (z == n2 ? pv : Rr[t]);
+            }
+        }
+        (void)z_shape;
+        (void)y_shape;
+    };
+    T Um[NU], U0[NU], Up[NU], UL[NU], F0[NU], FL[NU], R2[NRr], R1[NRr], R0[NRr];
+    const int64_t i0 = 2 * Ia - 1;  // first fine plane whose r is needed
+    load_u(i0 - 1, Um);
+    load_u(i0, U0);
+    load_u(i0 + 1, Up);
+    load_f(i0, F0);
+#pragma unroll
+    for (int t = 0; t < NRr; t++) R1[t] = R0[t] = (T)0;
+    const uint32_t cplane = (uint32_t)(((c1 - 1) * A.c.s1 + c2) * (int64_t)sizeof(T));
+    const int64_t K = K0 + (lane - 2) / 2;
+    const bool k_out = lane >= 2 && lane <= 60 && (lane & 1) == 0 && K < c2;
+    const uint32_t ckoff = (uint32_t)(min(max(K, (int64_t)0), c2 - 1) * (int64_t)sizeof(T));
+    for (int64_t i = i0; i <= 2 * Ib - 1; i++) {
+        load_u(i + 2, UL);
+        load_f(i + 1, FL);
+        // r(i): planes outside the shape are never emitted directly (clamped below)
+#pragma unroll
+        for (int t = 0; t < NRr; t++) R2[t] = R1[t], R1[t] = R0[t];
+        r_plane(i, Um, U0, Up, F0, R0);
+        if ((i & 1) && i >= 2 * Ia + 1) {
+            // coarse plane I = (i-1)/2 from r(2I-1) = R2, r(2I) = R1, r(2I+1) = R0,
+            // with 2I-1 < 0 and 2I+1 >= n0 clamped to 2I
+            const int64_t I = (i - 1) / 2;
+            const bool lo_clamp = 2 * I - 1 < 0, hi_clamp = 2 * I + 1 >= n0;
+            const __amdgpu_buffer_rsrc_t co = plane_rsrc(A.c.p + I * A.c.s0, cplane);
+#pragma unroll
+            for (int jr = 0; jr < RJ; jr++) {
+                const int64_t J = J0 + jr;
+                const int sm = 2 * jr, s0r = 2 * jr + 1, sp = 2 * jr + 2;
+                const T* Pm = lo_clamp ? R1 : R2;
+                const T* Pp = hi_clamp ? R1 : R0;
+                auto A3 = [&](const T* P, int srow, int dz) -> T {
+                    const T v = P[srow];
+                    return dz < 0 ? lane_prev(v) : (dz > 0 ? lane_next(v) : v);
+                };
+#define A_(P, srow, dz) A3(P, srow, dz)
+                T tmp = 1.0 * A_(Pm, sm, -1) + 2.0 * A_(Pm, s0r, -1) + 1.0 * A_(Pm, sp, -1) + 2.0 * A_(R1, sm, -1) +
+                        4.0 * A_(R1, s0r, -1) + 2.0 * A_(R1, sp, -1) + 1.0 * A_(Pp, sm, -1) + 2.0 * A_(Pp, s0r, -1) +
+                        1.0 * A_(Pp, sp, -1);
+                tmp += 2.0 * A_(Pm, sm, 0) + 4.0 * A_(Pm, s0r, 0) + 2.0 * A_(Pm, sp, 0) + 4.0 * A_(R1, sm, 0) +
+                       8.0 * A_(R1, s0r, 0) + 4.0 * A_(R1, sp, 0) + 2.0 * A_(Pp, sm, 0) + 4.0 * A_(Pp, s0r, 0) +
+                       2.0 * A_(Pp, sp, 0);
+                tmp += 1.0 * A_(Pm, sm, 1) + 2.0 * A_(Pm, s0r, 1) + 1.0 * A_(Pm, sp, 1) + 2.0 * A_(R1, sm, 1) +
+                       4.0 * A_(R1, s0r, 1) + 2.0 * A_(R1, sp, 1) + 1.0 * A_(Pp, sm, 1) + 2.0 * A_(Pp, s0r, 1) +
+                       1.0 * A_(Pp, sp, 1);
+#undef A_
+                const T out = tmp / 64.0;
+                const uint32_t jo = (uint32_t)(min(J, c1 - 1) * A.c.s1 * (int64_t)sizeof(T));
+                bstore(out, co, (k_out && J < c1) ? ckoff : BUF_OOB, jo);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < NU; t++) {
+            Um[t] = U0[t];
+            U0[t] = Up[t];
+            Up[t] = UL[t];
+            F0[t] = FL[t];
+        }
+    }
+}
+
+template <class T, int RJ>
+__global__ void __launch_bounds__(64) k_resid_fw_wave(RFArgs<T> A) {
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    // every fine position the tile touches inside the box and the shape
+    const int64_t J0 = (int64_t)ty * RJ, K0 = (int64_t)tz * FWK;
+    const int64_t Ia = (int64_t)xc * A.ichunk, Ib = min(A.c.n0, Ia + A.ichunk);
+    const bool inner = 2 * J0 - 1 >= A.by0 && 2 * (J0 + RJ) - 1 < A.by1 && 2 * K0 - 1 >= A.bz0 &&
+                       2 * K0 + 2 * FWK - 1 < A.bz1 && 2 * Ia - 1 >= A.bx0 && 2 * Ib - 1 < A.bx1 &&
+                       J0 + RJ <= A.c.n1 && K0 + FWK <= A.c.n2;
+    if (inner) resid_fw_march<T, RJ, false>(A, tl);
+    else resid_fw_march<T, RJ, true>(A, tl);
+}
+
 }  // namespace
 
 template <class T>
@@ -889,6 +1172,82 @@ template int march_residual<float>(int, const mgp_array*, const mgp_array*, cons
                                    double, double*, void*);
 template int march_residual_to<double, float>(int, const mgp_array*, const mgp_array*, const mgp_array*,
                                               const mgp_array*, const int64_t[6], double, double*, void*);
+
+
+// u += P(coarse) through wave tiles; MGP_ERR_ARG when the planes exceed the
+// 32-bit buffer range
+template <class T>
+int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stream) {
+    ProlongArgs<T> A;
+    A.c = view_of<T>(*coarse);
+    A.u = view_of<T>(*fine);
+    const int64_t fp = ((A.u.n1 - 1) * A.u.s1 + A.u.n2) * (int64_t)sizeof(T);
+    const int64_t cp = ((A.c.n1 - 1) * A.c.s1 + A.c.n2) * (int64_t)sizeof(T);
+    if (fp >= (int64_t)BUF_OOB || cp >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    constexpr int R = 16;
+    const int64_t nx = A.u.n0;
+    A.ytiles = (int)((A.u.n1 + R - 1) / R);
+    A.ztiles = (int)((A.u.n2 + 63) / 64);
+    const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+    int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 2)));
+    int64_t xchunk = (nx + chunks - 1) / chunks;
+    xchunk += xchunk & 1;  // chunks start on even planes
+    A.xchunk = xchunk;
+    A.ntiles = (int)(tiles * ((nx + xchunk - 1) / xchunk));
+    A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_prolong_add_wave<T, R>), dim3(A.ntiles_pad), dim3(64), 0, (hipStream_t)stream, A);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+template int march_prolong_add<double>(const mgp_array*, const mgp_array*, void*);
+template int march_prolong_add<float>(const mgp_array*, const mgp_array*, void*);
+int64_t g_prolong_kernel = 2;  // 1: per-point kernel, 2: wave tiles
+
+
+// coarse f = FW(f - L u) in one pass (consistent mode); MGP_ERR_ARG when the
+// layouts do not allow it (then the caller runs the residual and the
+// restriction separately)
+template <class T>
+int march_residual_fw(const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* cf,
+                      const int64_t b[6], double h, void* stream) {
+    for (int d = 0; d < 3; d++)
+        if (u->stride[d] != f->stride[d] || u->stride[d] != r->stride[d] || (u->shape[d] - 1) / 2 + 1 != cf->shape[d])
+            return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (u->halo[q] != f->halo[q] || u->halo[q] != r->halo[q]) return MGP_ERR_ARG;
+    RFArgs<T> A;
+    A.u = view_of<T>(*u);
+    A.f = view_of<T>(*f);
+    A.r = view_of<T>(*r);
+    A.c = view_of<T>(*cf);
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.xlo = -u->halo[MGP_WEST], A.xhi = u->shape[0] - 1 + u->halo[MGP_EAST];
+    A.ylo = -u->halo[MGP_SOUTH], A.yhi = u->shape[1] - 1 + u->halo[MGP_NORTH];
+    A.zlo = -u->halo[MGP_BOTTOM], A.zhi = u->shape[2] - 1 + u->halo[MGP_TOP];
+    const int64_t plane_bytes = ((A.yhi - A.ylo) * u->stride[1] + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(T);
+    const int64_t cplane = ((cf->shape[1] - 1) * cf->stride[1] + cf->shape[2]) * (int64_t)sizeof(T);
+    if (plane_bytes >= (int64_t)BUF_OOB || cplane >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    A.hsq = (T)(h * h);  // include/residual.h:23
+    A.inv_h2 = (T)(1.0 / (h * h));
+    constexpr int RJ = 4;
+    const int64_t c0 = cf->shape[0];
+    A.ytiles = (int)((cf->shape[1] + RJ - 1) / RJ);
+    A.ztiles = (int)((cf->shape[2] + FWK - 1) / FWK);
+    const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+    int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, c0 / 2));
+    A.ichunk = (c0 + chunks - 1) / chunks;
+    A.ntiles = (int)(tiles * ((c0 + A.ichunk - 1) / A.ichunk));
+    A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    hipLaunchKernelGGL((k_resid_fw_wave<T, RJ>), dim3(A.ntiles_pad), dim3(64), 0, (hipStream_t)stream, A);
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+template int march_residual_fw<double>(const mgp_array*, const mgp_array*, const mgp_array*, const mgp_array*,
+                                       const int64_t[6], double, void*);
+template int march_residual_fw<float>(const mgp_array*, const mgp_array*, const mgp_array*, const mgp_array*,
+                                      const int64_t[6], double, void*);
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
 int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch

@@ -42,7 +42,7 @@ class SolverConfig(ctypes.Structure):
                 ("origin", ctypes.c_double * 3), ("bc", ctypes.c_int32 * 6), ("dtype", ctypes.c_int32),
                 ("restriction", ctypes.c_int32), ("relaxation", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flags", ctypes.c_int32), ("maxiter", ctypes.c_int64), ("miniter", ctypes.c_int64),
-                ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64)]
+                ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64), ("refine_polish", ctypes.c_int64)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -113,6 +113,7 @@ for sfx in ("f64", "f32"):
         "mgp_rbgs_half_" + sfx: [i32, P(Array), P(Array), BOX, d, d, i32, vp],
         "mgp_rbgs_sweep_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, vp],
         "mgp_rbgs_sweeps_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
+        "mgp_residual_restrict_fw_" + sfx: [i32, P(Array), P(Array), P(Array), P(Array), BOX, d, vp],
         "mgp_jacobi_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, d, vp],
         "mgp_residual_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, vp, vp],
         "mgp_restrict_inject_" + sfx: [P(Array), P(Array), vp], "mgp_restrict_fw_" + sfx: [P(Array), P(Array), vp],
@@ -262,7 +263,7 @@ class Solver:
 
     def __init__(self, dims, levels, h, origin=(0.0, 0.0, 0.0), bc="DDDDDD", mode=REFERENCE,
                  relaxation=GAUSS_SEIDEL, restriction=INJECTION, problem=0, dtype=F64, flags=QUIET,
-                 maxiter=100, miniter=1, tolerance=1e-6, coarse_sweeps=0):
+                 maxiter=100, miniter=1, tolerance=1e-6, coarse_sweeps=0, refine_polish=0):
         require_gpu()
         c = SolverConfig()
         c.dims[:] = [int(x) for x in dims]
@@ -274,6 +275,7 @@ class Solver:
         c.mode, c.flags = int(mode), int(flags)
         c.maxiter, c.miniter, c.tolerance = int(maxiter), int(miniter), float(tolerance)
         c.coarse_sweeps = int(coarse_sweeps)
+        c.refine_polish = int(refine_polish)
         self.config = c
         self.levels = int(levels)
         self.np_dtype = np.float32 if dtype == F32 else np.float64

@@ -128,6 +128,12 @@ public:
     void cycle(const std::string solver, const int_t nsmooth = 10, T omega = -1.0);
     // residual on level 0 and ||r||inf / ||f||inf (waits)
     double relres();
+    // nsmooth smoothing passes on the finest level (enqueued)
+    void smooth_finest(int_t nsmooth, T omega) {
+        mgpx::StreamScope scope(stream);
+        if (omega < 0) omega = relaxation.default_omega();
+        relaxation.relax_level(*domains[0][0], omega, nsmooth);
+    }
     // enqueue on `s` (owned by the caller) instead of this solver's own stream;
     // recorded cycle graphs are dropped
     void use_stream(void* s) {
@@ -147,14 +153,14 @@ public:
 template <class T, template <class> class R, template <class> class P, template <class> class S>
 PoissonSolver<T, R, P, S>::PoissonSolver(int_t _dev_shape[3], Settings& _settings, BoundaryCondition& _BC)
     : num_devices(_dev_shape[0] * _dev_shape[1] * _dev_shape[2]), settings(_settings), BC(_BC),
-      grid(_settings, _settings.levels) {
+      grid(_settings, _settings.levels, _settings.print_settings) {
     for (int d = 0; d < 3; d++) dev_shape[d] = _dev_shape[d];
     this->alloc();
 }
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>
 PoissonSolver<T, R, P, S>::PoissonSolver(Settings& _settings, BoundaryCondition& _BC)
-    : num_devices(1), settings(_settings), BC(_BC), grid(_settings, _settings.levels) {
+    : num_devices(1), settings(_settings), BC(_BC), grid(_settings, _settings.levels, _settings.print_settings) {
     dev_shape[0] = dev_shape[1] = dev_shape[2] = 1;
     this->alloc();
 }
@@ -163,7 +169,7 @@ PoissonSolver<T, R, P, S>::PoissonSolver(Settings& _settings, BoundaryCondition&
 template <class T, template <class> class R, template <class> class P, template <class> class S>
 PoissonSolver<T, R, P, S>::PoissonSolver(int_t _dev_shape[3], Settings& _settings)
     : num_devices(_dev_shape[0] * _dev_shape[1] * _dev_shape[2]), settings(_settings),
-      BC({NEUMANN, NEUMANN, NEUMANN, NEUMANN, DIRICHLET, DIRICHLET}), grid(_settings, _settings.levels) {
+      BC({NEUMANN, NEUMANN, NEUMANN, NEUMANN, DIRICHLET, DIRICHLET}), grid(_settings, _settings.levels, _settings.print_settings) {
     for (int d = 0; d < 3; d++) dev_shape[d] = _dev_shape[d];
     this->alloc();
 }

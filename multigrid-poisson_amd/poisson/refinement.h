@@ -36,6 +36,7 @@ public:
         outer_settings.levels = 1;
         inner_settings.print_cycle_time = false;
         outer_settings.print_cycle_time = false;
+        inner_settings.print_settings = false;  // the FP64 levels print, as one solver would
         outer = new PoissonSolver<double, R, P, S>(outer_settings, BC);
         try {
             inner = new PoissonSolver<float, R, P, S>(inner_settings, BC);
@@ -142,8 +143,12 @@ public:
         inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
         const mgp_array e32 = d32.u->desc();
         mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
+        // optional FP64 smoothing of the updated solution: settles u at a
+        // fixed point of the FP64 smoother, whose residual sits below the
+        // rounding floor of an unsmoothed FP64 u at fine h
+        if (settings.refine_polish > 0) outer->smooth_finest(settings.refine_polish, omega);
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
-        residual_norm_to_f32(d64, f32, dev_norm);
+        residual_norm_to_f32(outer->get_domain(0, 0), f32, dev_norm);
         mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
         f32_ready = true;
     }

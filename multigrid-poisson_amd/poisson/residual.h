@@ -18,6 +18,12 @@ inline void refresh_boundaries(Domain<T>& domain, bool fetch_neighbor) {
 }
 }  // namespace detail
 
+// the boundary refresh residual() performs before its kernel
+template <class T>
+inline void detail_refresh_for_residual(Domain<T>& domain) {
+    detail::refresh_boundaries(domain, true);
+}
+
 template <class T>
 void residual(Domain<T>& domain, bool fetch_neighbor) {
     detail::refresh_boundaries(domain, fetch_neighbor);

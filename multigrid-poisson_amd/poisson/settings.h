@@ -52,8 +52,11 @@ public:
     bool use_graphs = true;        // replay each cycle as one captured HIP graph
     bool fused_sweeps = true;      // one-pass red-black sweeps where the colouring allows
     bool print_cycle_time = true;  // the reference prints "<cycle> took <s> seconds" per cycle
+    bool print_settings = true;    // the reference prints every level's DomainSettings at construction
     int_t coarse_sweeps = 0;       // CONSISTENT only: smoothing passes per visit of the coarsest
                                    // level (0 = nsmooth, the reference's schedule)
+    int_t refine_polish = 0;       // RefinementSolver: FP64 smoothing passes on the finest level
+                                   // after each update (0 = none)
 
     Settings() {}
     Settings(const Settings& s) = default;

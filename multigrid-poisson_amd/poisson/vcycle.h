@@ -17,8 +17,22 @@ void descend(Domain<T>*** domains, Restriction<T>& restriction, uint_t level, ui
     for (uint_t g = 0; g < num_devices; g++) {
         Domain<T>& d = *domains[g][level];
         Domain<T>& c = *domains[g][level + 1];
-        residual<T>(d);
-        restriction.restrict_to(*d.r, *c.f);
+        // consistent mode with fused kernels: residual and full weighting in
+        // one pass (the fine r is not needed afterwards: the prolongation is
+        // fused with the correction)
+        bool done = false;
+        if (d.settings.mode == CONSISTENT && d.settings.fused_sweeps && restriction.mgp_kind() == MGP_FULL_WEIGHTING) {
+            detail_refresh_for_residual(d);
+            int64_t b[6];
+            d.box(b);
+            const mgp_array u = d.u->desc(), f = d.f->desc(), r = d.r->desc(), cf = c.f->desc();
+            done = mgpx::Ops<T>::residual_restrict_fw(d.settings.mode, &u, &f, &r, &cf, b, d.settings.h,
+                                                      mgpx::stream()) == MGP_OK;
+        }
+        if (!done) {
+            residual<T>(d);
+            restriction.restrict_to(*d.r, *c.f);
+        }
         c.u->init_zero();
         c.north->restrict_to(*d.u, *d.north, d.settings, restriction);
         c.south->restrict_to(*d.u, *d.south, d.settings, restriction);

@@ -509,3 +509,71 @@ def test_lds_multi_sweep_rejects_large_levels():
     rc = mgp.lib().mgp_rbgs_sweeps_f64(mgp.CONSISTENT, u.ref(), u.ref(), u.ref(), ctypes.byref(F),
                                        mgp.box([1, 64, 1, 64, 1, 32]), 1.0, 3, None)
     assert rc == mgp.ERR_ARG
+
+
+PROLONG_SHAPES = [(1025 // 16 + 1, 33, 17), (17, 33, 65), (9, 9, 9), (33, 17, 129), (5, 3, 3)]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape", PROLONG_SHAPES, ids=[str(i) for i in range(len(PROLONG_SHAPES))])
+@pytest.mark.parametrize("halo", [(0, 0, 0, 0, 0, 0), (1, 1, 1, 1, 0, 0), (1, 0, 0, 1, 1, 0)], ids=["h0", "h1", "h2"])
+def test_wave_prolong_add_equals_point_kernel(shape, halo, dtype):
+    """u += P(coarse) through wave tiles (prolong.kernel 2) == the per-point
+    kernel (prolong.kernel 1), bit for bit, over the whole fine shape."""
+    say("case prolong", shape, halo, dtype.__name__)
+    L = mgp.lib()
+    cshape = tuple((n - 1) // 2 + 1 for n in shape)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, halo)).encode()))
+    c0 = rng.standard_normal(mgp.padded_shape(list(cshape), list(halo)))
+    u0 = rng.standard_normal(mgp.padded_shape(list(shape), list(halo)))
+    sfx = "f64" if dtype == np.float64 else "f32"
+    out = {}
+    for k in (1, 2):
+        mgp.check(L.mgp_tuning_set(b"prolong.kernel", k))
+        c = mgp.DeviceArray(cshape, halo, dtype=dtype).upload(c0)
+        u = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+        mgp.call("mgp_prolong_add_" + sfx, c.ref(), u.ref(), None)
+        out[k] = u.download()
+    mgp.check(L.mgp_tuning_set(b"prolong.kernel", 2))
+    np.testing.assert_array_equal(out[1], out[2])
+
+
+RFW_SHAPES = [((65, 33, 129), "NNNNDD"), ((17, 33, 65), "DDDDDD"), ((33, 17, 65), "NDDNND"), ((9, 17, 129), "DNNDDN"),
+              ((5, 5, 5), "NNNNNN"), ((129, 9, 17), "DDNNDN")]
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,bc", RFW_SHAPES, ids=[str(i) for i in range(len(RFW_SHAPES))])
+def test_fused_residual_restriction_equals_separate(shape, bc, dtype):
+    """mgp_residual_restrict_fw (residual in registers, full weighting of it,
+    r outside the box read as stored) == mgp_residual then mgp_restrict_fw,
+    bit for bit over the whole coarse shape, consistent mode."""
+    say("case resid-fw", shape, bc, dtype.__name__)
+    halo = [1 if c == "N" else 0 for c in bc]
+    cshape = tuple((n - 1) // 2 + 1 for n in shape)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc)).encode()))
+    sfx = "f64" if dtype == np.float64 else "f32"
+    u = mgp.DeviceArray(shape, halo, dtype=dtype).upload(rng.standard_normal(mgp.padded_shape(list(shape), halo)))
+    f = mgp.DeviceArray(shape, halo, dtype=dtype).upload(rng.standard_normal(mgp.padded_shape(list(shape), halo)))
+    r0 = rng.standard_normal(mgp.padded_shape(list(shape), halo))
+    c0 = rng.standard_normal(mgp.padded_shape(list(cshape), halo))
+    box = [int(bc[3] == "D"), shape[0] - int(bc[2] == "D"), int(bc[1] == "D"), shape[1] - int(bc[0] == "D"),
+           int(bc[5] == "D"), shape[2] - int(bc[4] == "D")]
+    h = 1.0 / (shape[0] - 1)
+    r = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+    c = mgp.DeviceArray(cshape, halo, dtype=dtype).upload(c0)
+    mgp.call("mgp_residual_" + sfx, mgp.CONSISTENT, u.ref(), f.ref(), r.ref(), mgp.box(box), h, None, None)
+    mgp.call("mgp_restrict_fw_" + sfx, r.ref(), c.ref(), None)
+    want = c.download()
+    r2 = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+    c2 = mgp.DeviceArray(cshape, halo, dtype=dtype).upload(c0)
+    mgp.call("mgp_residual_restrict_fw_" + sfx, mgp.CONSISTENT, u.ref(), f.ref(), r2.ref(), c2.ref(), mgp.box(box), h,
+             None)
+    got = c2.download()
+    hN, hS, hE, hW, hT, hB = halo
+    sl = (slice(hW, hW + cshape[0]), slice(hS, hS + cshape[1]), slice(hB, hB + cshape[2]))
+    np.testing.assert_array_equal(got[sl], want[sl])
+    # r itself is left as it was
+    np.testing.assert_array_equal(r2.download(), r0.astype(dtype))
+    assert mgp.lib().mgp_residual_restrict_fw_f64(mgp.REFERENCE, u.ref(), f.ref(), r2.ref(), c2.ref(), mgp.box(box), h,
+                                                  None) == mgp.ERR_ARG
