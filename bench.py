@@ -84,6 +84,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--batch-count", type=int, default=256)
     ap.add_argument("--serve-count", type=int, default=48)
+    ap.add_argument("--serve-workers", type=int, default=12, help="solves in flight in the serving leg")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run once each: a fill (write-only) and an inf-norm (read-only) of one 4.3 GB array, "
@@ -236,30 +237,40 @@ def interior(a, shape, halo):
 
 
 def mixed_leg(dims, h, args):
-    """FP32 cycles inside FP64 refinement vs plain FP64, all-Dirichlet,
-    u = cos(x z^2) sin(y^3), same grid, levels, cycle and tolerance."""
+    """Config 3: FP32 cycles inside FP64 refinement vs plain FP64, same grid,
+    levels, F-cycle and coarsest passes.  Problem of config 3 (all-Dirichlet,
+    u = cos(x z^2) sin(y^3)) at tol 1e-10 and at 2e-10, and the headline
+    problem (Neumann x/y, u = x^3 y^2 z) at 1e-10; max-abs differences of the
+    config-3 solutions from each other and from the analytic u."""
     import numpy as np
 
     dof = dims[0] * dims[1] * dims[2]
-    out = {"workload": "poisson-fcycle-mixed-%dx%dx%d" % tuple(dims), "bc": "Dirichlet all faces",
-           "u": "cos(x z^2) sin(y^3)"}
+    out = {"workload": "poisson-fcycle-mixed-%dx%dx%d" % tuple(dims)}
     sols = {}
-    for name, dt in (("fp64", mgp.F64), ("fp32_refine", mgp.MIXED)):
-        log("bench: mixed leg, %s solver" % name)
-        s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), "DDDDDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
-                       mgp.FULL_WEIGHTING, problem=3, dtype=dt,
-                       flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0), maxiter=100, miniter=1,
-                       tolerance=args.tol, coarse_sweeps=args.coarse_sweeps)
-        s.to_device()
-        el, cyc, rel = timed_solves(s, args)
-        out[name] = {"DOF/s": dof * args.steps / el, "ms_per_solve": 1e3 * el / args.steps, "cycles": cyc[-1],
-                     "final_rel_res": rel[-1]}
-        log("bench:   %.1f ms/solve, %d cycles, rel_res %.3e" % (1e3 * el / args.steps, cyc[-1], rel[-1]))
-        s.to_host()
-        host, a = s._arr(0, "u")
-        sols[name] = np.array(interior(s.array("u"), list(a.shape), list(a.halo)), copy=True)
-        s.close()
-    out["speedup"] = out["fp32_refine"]["DOF/s"] / out["fp64"]["DOF/s"]
+    cases = (("cos_dirichlet", 3, "DDDDDD", 1e-10), ("cos_dirichlet", 3, "DDDDDD", 2e-10),
+             ("cubic_neumann_xy", 2, "NNNNDD", 1e-10))
+    for label, prob, bc, tol in cases:
+        key = "%s_tol%.0e" % (label, tol)
+        out[key] = {}
+        for name, dt in (("fp64", mgp.F64), ("fp32_refine", mgp.MIXED)):
+            log("bench: mixed leg, %s %s" % (key, name))
+            s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                           mgp.FULL_WEIGHTING, problem=prob, dtype=dt,
+                           flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0), maxiter=30, miniter=1,
+                           tolerance=tol, coarse_sweeps=args.coarse_sweeps)
+            s.to_device()
+            el, cyc, rel = timed_solves(s, args)
+            out[key][name] = {"DOF/s": dof * args.steps / el, "ms_per_solve": 1e3 * el / args.steps,
+                              "iterations": cyc[-1], "final_rel_res": rel[-1], "reached_tol": rel[-1] <= tol}
+            log("bench:   %.1f ms/solve, %d iterations, rel_res %.3e" % (1e3 * el / args.steps, cyc[-1], rel[-1]))
+            if prob == 3 and tol == 1e-10:
+                s.to_host()
+                host, a = s._arr(0, "u")
+                sols[name] = np.array(interior(s.array("u"), list(a.shape), list(a.halo)), copy=True)
+            s.close()
+        o = out[key]
+        o["speedup"] = o["fp32_refine"]["DOF/s"] / o["fp64"]["DOF/s"]
+        o["ms_per_iteration"] = {n: o[n]["ms_per_solve"] / max(o[n]["iterations"], 1) for n in ("fp64", "fp32_refine")}
     a, b = sols["fp32_refine"], sols["fp64"]
     x = np.arange(dims[0]) * h
     y = np.arange(dims[1]) * h
@@ -272,43 +283,10 @@ def mixed_leg(dims, h, args):
         d_fp64 = max(d_fp64, float(np.max(np.abs(a[i0:i0 + 64] - b[i0:i0 + 64]))))
         err_mixed = max(err_mixed, float(np.max(np.abs(a[i0:i0 + 64] - ua))))
         err_fp64 = max(err_fp64, float(np.max(np.abs(b[i0:i0 + 64] - ua))))
-    out["max_abs_diff_vs_fp64"] = d_fp64
-    out["max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp64": err_fp64}
-    log("bench: mixed %.2fx FP64, |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" %
-        (out["speedup"], d_fp64, err_mixed, err_fp64))
+    out["cos_dirichlet_max_abs_diff_vs_fp64"] = d_fp64
+    out["cos_dirichlet_max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp64": err_fp64}
+    log("bench: mixed |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" % (d_fp64, err_mixed, err_fp64))
     return out
-
-
-def traffic_probe(args):
-    """Three kernels for HBM-counter calibration: fill of u (8 B/point written),
-    inf-norm of f (8 B/point read), one one-pass sweep u -> r (24 B/box point)."""
-    import numpy as np
-
-    n, nz = args.n, args.n // 2 + 1
-    h = 1.0 / (n - 1)
-    s = mgp.Solver((n, n, nz), args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
-                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET | mgp.NO_GRAPHS, coarse_sweeps=args.coarse_sweeps)
-    s.to_device()
-    L = mgp.lib()
-    u, f_, r = s.device_array("u"), s.device_array("f"), s.device_array("r")
-    scal = mgp.DeviceBuffer(np.zeros(1))
-    F = mgp.Faces()
-    for q in range(6):
-        t = s.face_type(q)
-        F.type[q] = t
-        host, a, tt = mgp.vp(), mgp.Array(), ctypes.c_int()
-        mgp.check(L.mgp_solver_face(s.h, 0, q, ctypes.byref(host), ctypes.byref(a), ctypes.byref(tt)))
-        F.g[q] = a.data if t == mgp.NEUMANN else None
-    F.h = h
-    L.mgp_solver_sync(s.h)
-    mgp.check(L.mgp_fill_f64(ctypes.byref(r), 0.0, None))
-    mgp.check(L.mgp_norm_inf_f64(ctypes.byref(f_), scal.ptr, None))
-    mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_), ctypes.byref(F),
-                                   mgp.box([0, n, 0, n, 1, nz - 1]), 1.0, None))
-    L.mgp_stream_sync(None)
-    pts = n * n * nz
-    log("traffic probe: fill %d B, norm %d B, sweep %d B algorithmic" % (8 * pts, 8 * pts, 24 * n * n * (nz - 2)))
-    s.close()
 
 
 def guarded_leg(name, skip, fn):
@@ -439,7 +417,7 @@ def serve_leg(args):
     log("bench:   %.2f s, %.1f solves/s" % (seq_s, seq_rate))
 
     def run(arrival_rate):
-        srv = mgp.Server(hbm_budget=96 << 30, max_active=4, tolerance=tol, coarse_sweeps=32)
+        srv = mgp.Server(hbm_budget=96 << 30, max_active=args.serve_workers, tolerance=tol, coarse_sweeps=32)
         start = time.perf_counter()
         ids = []
         t_next = start
@@ -458,7 +436,10 @@ def serve_leg(args):
         lat = np.array([r.latency_s for r in res])
         bad = [r for r in res if r.status != 0 or r.rel_res > tol]
         return {"solves/s": len(reqs) / wall, "p50_latency_s": float(np.percentile(lat, 50)),
-                "p99_latency_s": float(np.percentile(lat, 99)), "peak_active": peak, "failed": len(bad)}
+                "p99_latency_s": float(np.percentile(lat, 99)), "peak_active": peak, "failed": len(bad),
+                "mean_setup_s": float(np.mean([r.setup_s for r in res])),
+                "mean_run_s": float(np.mean([r.run_s for r in res])),
+                "mean_fetch_s": float(np.mean([r.fetch_s for r in res]))}
 
     log("bench:   server burst")
     burst = run(None)
@@ -467,6 +448,7 @@ def serve_leg(args):
     poisson = run(rate)
     poisson["arrival_rate"] = rate
     return {"workload": "serving-%d-requests-2^k+1-mixed-bc" % len(reqs), "tolerance": tol,
+            "workers": args.serve_workers,
             "one_at_a_time_fp64_solves/s": seq_rate, "burst": burst, "poisson": poisson,
             "capacity_vs_fp64_one_at_a_time": burst["solves/s"] / seq_rate}
 

@@ -265,6 +265,16 @@ int mgp_solver_save(void* solver, const char* path);
 /* message of the last failing mgp_solver_* / mgp_batch_* call on this thread */
 const char* mgp_last_error(void);
 
+/* max |f - L u| over the box, and r's stored values outside it, folded into
+ * *norm (a device double the caller zeroed); r is not written.  The
+ * convergence check of a solve, which needs the norm only (the solve writes
+ * r once, for the final u).  MGP_ERR_ARG when the layouts do not allow the
+ * one-pass form. */
+int mgp_residual_norm_f64(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
+                          double h, double* norm, void* stream);
+int mgp_residual_norm_f32(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
+                          double h, double* norm, void* stream);
+
 /* Consistent mode: coarse f = full weighting (include/fullweighting.h:59-121)
  * of r = f - L u (include/residual.h:58-79), r computed on the box and not
  * stored, r outside the box read as stored; u's halos must hold current
@@ -283,6 +293,11 @@ int mgp_residual_restrict_fw_f32(int mode, const mgp_array* u, const mgp_array* 
  * one pass (no counterpart in the reference). */
 int mgp_residual_cast_f64_f32(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r,
                               const mgp_array* out, const int64_t box[6], double h, double* norm, void* stream);
+
+/* the shape of a (halos dropped) into the dense C-order device buffer `out`
+ * (shape[0]*shape[1]*shape[2] doubles); a result copy needs one transfer then
+ * (no counterpart in the reference) */
+int mgp_pack_f64(const mgp_array* a, void* out, void* stream);
 
 /* synthetic right-hand side over the shape of f: sum over nmodes (<= 16) of
  * amp[m] sin(a pi x) sin(b pi y) sin(c pi z), (a, b, c) = abc[3m..3m+2], at

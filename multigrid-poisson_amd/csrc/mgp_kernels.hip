@@ -153,6 +153,14 @@ __global__ void k_fill_modes(View<T> f, Box bx, int64_t zc, Modes m, double h) {
     f.at(i, j, k) = (T)s;
 }
 
+// the shape of a halo-padded array into a dense C-order buffer
+template <class T>
+__global__ void k_pack(View<T> a, Box bx, int64_t zc, T* out) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    out[(i * a.n1 + j) * a.n2 + k] = a.at(i, j, k);
+}
+
 __global__ void k_zero_scalars(double* p, int64_t n) {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) p[t] = 0.0;
@@ -851,6 +859,21 @@ int mgp_residual_restrict_fw_f32(int m, const mgp_array* u, const mgp_array* f, 
     return residual_restrict_fw<float>(m, u, f, r, cf, b, h, s);
 }
 
+int mgp_residual_norm_f64(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6],
+                          double h, double* norm, void* s) {
+    if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !norm || !same_shape(u, f) || !same_shape(u, r) ||
+        !valid_box(u, b) || (m != MGP_MODE_REFERENCE && m != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    return march_residual_to<double, double>(m, u, f, r, r, b, h, norm, s, false);
+}
+int mgp_residual_norm_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6],
+                          double h, double* norm, void* s) {
+    if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !norm || !same_shape(u, f) || !same_shape(u, r) ||
+        !valid_box(u, b) || (m != MGP_MODE_REFERENCE && m != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    return march_residual_to<float, float>(m, u, f, r, r, b, h, norm, s, false);
+}
+
 int mgp_residual_cast_f64_f32(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,
                               const mgp_array* out, const int64_t b[6], double h, double* norm, void* s) {
     if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !valid_array(out) || !same_shape(u, f) ||
@@ -858,6 +881,15 @@ int mgp_residual_cast_f64_f32(int m, const mgp_array* u, const mgp_array* f, con
         (m != MGP_MODE_REFERENCE && m != MGP_MODE_CONSISTENT))
         return MGP_ERR_ARG;
     return march_residual_to<double, float>(m, u, f, r, out, b, h, norm, s);
+}
+
+int mgp_pack_f64(const mgp_array* a, void* out, void* s) {
+    if (!valid_array(a) || !out) return MGP_ERR_ARG;
+    const Box bx = shape_box(a);
+    const BoxGrid g = box_grid(bx);
+    hipLaunchKernelGGL(k_pack<double>, g.grid, g.block, 0, (hipStream_t)s, view_of<double>(*a), bx, g.zchunks,
+                       static_cast<double*>(out));
+    return launch_status();
 }
 
 int mgp_zero_scalars(double* p, int64_t n, void* s) {

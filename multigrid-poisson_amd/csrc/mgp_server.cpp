@@ -76,7 +76,6 @@ struct SlotBase {
     Key key;
     size_t bytes = 0;
     Job* job = nullptr;
-    std::vector<double> staging;  // host copy of the padded solution
     virtual ~SlotBase() {}
     virtual void begin(const mgp_request& r) = 0;  // f, u = 0 (waits)
     virtual void run() = 0;                         // the solve (waits)
@@ -122,20 +121,16 @@ struct Slot : SlotBase {
     }
     void run() override { solver->solve("Fcycle", nsmooth, omega, "", 1e30); }
     void finish(double* out) override {
-        const mgp_array u = solver->fp64().get_domain(0, 0).u->desc();
-        const int64_t p0 = u.shape[0] + u.halo[MGP_EAST] + u.halo[MGP_WEST];
-        const int64_t n = p0 * u.stride[0];
-        staging.resize(n);
-        hip_check(hipMemcpyAsync(staging.data(), u.data, n * sizeof(double), hipMemcpyDeviceToHost,
-                                 (hipStream_t)solver->get_stream()),
-                  "server result");
-        hip_check(hipStreamSynchronize((hipStream_t)solver->get_stream()), "server result");
         if (!out) return;
-        const int64_t base = u.halo[MGP_WEST] * u.stride[0] + u.halo[MGP_SOUTH] * u.stride[1] + u.halo[MGP_BOTTOM];
-        for (int64_t i = 0; i < u.shape[0]; i++)
-            for (int64_t j = 0; j < u.shape[1]; j++)
-                std::memcpy(out + (i * u.shape[1] + j) * u.shape[2], staging.data() + base + i * u.stride[0] + j * u.stride[1],
-                            u.shape[2] * sizeof(double));
+        // pack the interior into the (now free) FP64 residual array, then one copy
+        Poisson::Domain<double>& d = solver->fp64().get_domain(0, 0);
+        const mgp_array u = d.u->desc();
+        void* st = solver->get_stream();
+        Poisson::mgpx::check(mgp_pack_f64(&u, d.r->devptr, st), "server result");
+        const size_t n = (size_t)(u.shape[0] * u.shape[1] * u.shape[2]);
+        hip_check(hipMemcpyAsync(out, d.r->devptr, n * sizeof(double), hipMemcpyDeviceToHost, (hipStream_t)st),
+                  "server result");
+        hip_check(hipStreamSynchronize((hipStream_t)st), "server result");
     }
     void stats(int64_t* iters, double* rel) override {
         *iters = (int64_t)solver->solve_iterations();

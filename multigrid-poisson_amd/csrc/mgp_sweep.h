@@ -17,9 +17,10 @@ template <class T>
 int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6],
                    double h, double* norm, void* stream);
 // the same with r written, in type O, to `out` (same element layout) instead of r
+// (store = false: only the norm; r and out untouched)
 template <class T, class O>
 int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
-                      const int64_t box[6], double h, double* norm, void* stream);
+                      const int64_t box[6], double h, double* norm, void* stream, bool store = true);
 // u += trilinear P(coarse) over the fine shape through wave tiles
 template <class T>
 int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stream);

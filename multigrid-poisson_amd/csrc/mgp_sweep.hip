@@ -634,7 +634,7 @@ __device__ __forceinline__ T resid_point(T uc, T xm, T xp, T ym, T yp, T zm, T z
     }
 }
 
-template <class T, class O, int MODE, int R, bool NORM>
+template <class T, class O, int MODE, int R, bool NORM, bool STORE = true>
 __global__ void __launch_bounds__(64) k_residual_wave(ResidArgs<T, O> A) {
     constexpr int NR = R + 2;
     const int per_xcd = A.ntiles_pad / 8;
@@ -695,8 +695,9 @@ __global__ void __launch_bounds__(64) k_residual_wave(ResidArgs<T, O> A) {
             const T rv = resid_point<T, MODE>(U0[t], Um1[t], Up1[t], U0[t - 1], U0[t + 1], lane_prev(U0[t]),
                                               lane_next(U0[t]), F0[t], A.hsq, A.inv_h2);
             const bool out = lane_out && yout[t];
-            bstore((O)rv, ro, out ? (uint32_t)((uint64_t)zoff * sizeof(O) / sizeof(T)) : BUF_OOB,
-                   (uint32_t)((uint64_t)roff[t] * sizeof(O) / sizeof(T)));
+            if (STORE)
+                bstore((O)rv, ro, out ? (uint32_t)((uint64_t)zoff * sizeof(O) / sizeof(T)) : BUF_OOB,
+                       (uint32_t)((uint64_t)roff[t] * sizeof(O) / sizeof(T)));
             if (NORM) vmax = out ? fmax(vmax, fabs((double)rv)) : vmax;
         }
 #pragma unroll
@@ -1093,7 +1094,8 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
 // with the same element layout); the norm reads r outside the box.
 template <class T, class O>
 int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
-                      const int64_t b[6], double h, double* norm, void* stream) {
+                      const int64_t b[6], double h, double* norm, void* stream, bool store) {
+    if (!store && !norm) return MGP_ERR_ARG;
     for (int d = 0; d < 3; d++)
         if (u->stride[d] != f->stride[d] || u->stride[d] != r->stride[d] || u->stride[d] != out->stride[d] ||
             u->shape[d] != out->shape[d])
@@ -1131,7 +1133,10 @@ int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mg
 #define MGP_RES_LAUNCH(RR, NN)                                                                             \
     if (ref) hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_REFERENCE, RR, NN>), g, blk, 0, st, A); \
     else hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_CONSISTENT, RR, NN>), g, blk, 0, st, A);
-        if (norm) {
+        if (!store) {
+            if (ref) hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_REFERENCE, 16, true, false>), g, blk, 0, st, A);
+            else hipLaunchKernelGGL((k_residual_wave<T, O, MGP_MODE_CONSISTENT, 16, true, false>), g, blk, 0, st, A);
+        } else if (norm) {
             if (rows == 8) { MGP_RES_LAUNCH(8, true) } else { MGP_RES_LAUNCH(16, true) }
         } else {
             if (rows == 8) { MGP_RES_LAUNCH(8, false) } else { MGP_RES_LAUNCH(16, false) }
@@ -1160,7 +1165,7 @@ int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mg
 template <class T>
 int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6],
                    double h, double* norm, void* stream) {
-    return march_residual_to<T, T>(mode, u, f, r, r, b, h, norm, stream);
+    return march_residual_to<T, T>(mode, u, f, r, r, b, h, norm, stream, true);
 }
 
 int64_t g_resid_kernel = 2;  // 1: per-point kernels, 2: wave tiles
@@ -1171,7 +1176,11 @@ template int march_residual<double>(int, const mgp_array*, const mgp_array*, con
 template int march_residual<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const int64_t[6],
                                    double, double*, void*);
 template int march_residual_to<double, float>(int, const mgp_array*, const mgp_array*, const mgp_array*,
-                                              const mgp_array*, const int64_t[6], double, double*, void*);
+                                              const mgp_array*, const int64_t[6], double, double*, void*, bool);
+template int march_residual_to<double, double>(int, const mgp_array*, const mgp_array*, const mgp_array*,
+                                               const mgp_array*, const int64_t[6], double, double*, void*, bool);
+template int march_residual_to<float, float>(int, const mgp_array*, const mgp_array*, const mgp_array*,
+                                             const mgp_array*, const int64_t[6], double, double*, void*, bool);
 
 
 // u += P(coarse) through wave tiles; MGP_ERR_ARG when the planes exceed the

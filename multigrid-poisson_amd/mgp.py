@@ -97,7 +97,7 @@ _SIGS = {
     "mgp_solver_cycle": [vp, ctypes.c_char_p, i64, d], "mgp_solver_relres": [vp, P(d)], "mgp_solver_sync": [vp], "mgp_solver_reset": [vp],
     "mgp_solver_stats": [vp, P(d), P(d), P(i64)], "mgp_solver_array": [vp, i32, i32, P(vp), P(Array)],
     "mgp_solver_face": [vp, i32, i32, P(vp), P(Array), P(ctypes.c_int)], "mgp_solver_save": [vp, ctypes.c_char_p],
-    "mgp_fill_modes_f64": [P(Array), i64, P(ctypes.c_int32), P(d), d, vp],
+    "mgp_fill_modes_f64": [P(Array), i64, P(ctypes.c_int32), P(d), d, vp], "mgp_pack_f64": [P(Array), vp, vp],
     "mgp_residual_cast_f64_f32": [i32, P(Array), P(Array), P(Array), P(Array), BOX, d, vp, vp],
     "mgp_batch_create": [P(BatchConfig), P(i64), P(vp)], "mgp_batch_destroy": [vp],
     "mgp_batch_set_rhs": [vp, i64, P(d)], "mgp_batch_solve": [vp, P(d)], "mgp_batch_stats": [vp, i64, P(d), P(i64)],
@@ -114,6 +114,7 @@ for sfx in ("f64", "f32"):
         "mgp_rbgs_sweep_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, vp],
         "mgp_rbgs_sweeps_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
         "mgp_residual_restrict_fw_" + sfx: [i32, P(Array), P(Array), P(Array), P(Array), BOX, d, vp],
+        "mgp_residual_norm_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, vp, vp],
         "mgp_jacobi_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, d, vp],
         "mgp_residual_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, vp, vp],
         "mgp_restrict_inject_" + sfx: [P(Array), P(Array), vp], "mgp_restrict_fw_" + sfx: [P(Array), P(Array), vp],

@@ -72,6 +72,10 @@ template <class T> struct Ops;
                               const mgp_faces* F, const int64_t b[6], double w, void* s) {                       \
             return mgp_rbgs_sweep_##SFX(m, ui, uo, f, F, b, w, s);                                                \
         }                                                                                                         \
+        static int residual_norm(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,              \
+                                 const int64_t b[6], double h, double* n, void* s) {                            \
+            return mgp_residual_norm_##SFX(m, u, f, r, b, h, n, s);                                               \
+        }                                                                                                         \
         static int residual_restrict_fw(int m, const mgp_array* u, const mgp_array* f, const mgp_array* r,       \
                                         const mgp_array* c, const int64_t b[6], double h, void* s) {             \
             return mgp_residual_restrict_fw_##SFX(m, u, f, r, c, b, h, s);                                        \

@@ -45,6 +45,21 @@ void residual_norm(Domain<T>& domain, double* norm_dev, bool fetch_neighbor = tr
     mgpx::check(mgpx::Ops<T>::residual(domain.settings.mode, &u, &f, &r, b, domain.settings.h, norm_dev, mgpx::stream()),
                 "residual");
 }
+// addition: max |f - L u| into *norm_dev without writing r (r is written
+// with the same values when the one-pass form is not available)
+template <class T>
+void residual_norm_only(Domain<T>& domain, double* norm_dev) {
+    detail::refresh_boundaries(domain, true);
+    int64_t b[6];
+    domain.box(b);
+    const mgp_array u = domain.u->desc(), f = domain.f->desc(), r = domain.r->desc();
+    if (mgpx::Ops<T>::residual_norm(domain.settings.mode, &u, &f, &r, b, domain.settings.h, norm_dev,
+                                    mgpx::stream()) == MGP_OK)
+        return;
+    mgpx::check(mgpx::Ops<T>::residual(domain.settings.mode, &u, &f, &r, b, domain.settings.h, norm_dev, mgpx::stream()),
+                "residual");
+}
+
 // addition: the same residual on an FP64 level, written as FP32 into `out`
 // (same layout) instead of r, with max |r| into *norm_dev; r is only read
 // (outside the box, for the norm)

@@ -577,3 +577,29 @@ def test_fused_residual_restriction_equals_separate(shape, bc, dtype):
     np.testing.assert_array_equal(r2.download(), r0.astype(dtype))
     assert mgp.lib().mgp_residual_restrict_fw_f64(mgp.REFERENCE, u.ref(), f.ref(), r2.ref(), c2.ref(), mgp.box(box), h,
                                                   None) == mgp.ERR_ARG
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,bc", RESID_SHAPES[:4], ids=[str(i) for i in range(4)])
+def test_norm_only_residual(shape, bc, dtype):
+    """mgp_residual_norm (the solve loop's check) gives the norm of
+    mgp_residual with norm, bit for bit, and leaves r untouched."""
+    say("case norm-only", shape, bc, dtype.__name__)
+    halo = [1 if c == "N" else 0 for c in bc]
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc, "norm")).encode()))
+    sfx = "f64" if dtype == np.float64 else "f32"
+    u = mgp.DeviceArray(shape, halo, dtype=dtype).upload(rng.standard_normal(mgp.padded_shape(list(shape), halo)))
+    f = mgp.DeviceArray(shape, halo, dtype=dtype).upload(rng.standard_normal(mgp.padded_shape(list(shape), halo)))
+    r0 = rng.standard_normal(mgp.padded_shape(list(shape), halo)).astype(dtype)
+    box = [int(bc[3] == "D"), shape[0] - int(bc[2] == "D"), int(bc[1] == "D"), shape[1] - int(bc[0] == "D"),
+           int(bc[5] == "D"), shape[2] - int(bc[4] == "D")]
+    h = 1.0 / (shape[0] - 1)
+    for mode in (mgp.REFERENCE, mgp.CONSISTENT):
+        r1 = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+        n1 = mgp.DeviceBuffer(np.zeros(1))
+        mgp.call("mgp_residual_" + sfx, mode, u.ref(), f.ref(), r1.ref(), mgp.box(box), h, n1.ptr, None)
+        r2 = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
+        n2 = mgp.DeviceBuffer(np.zeros(1))
+        mgp.call("mgp_residual_norm_" + sfx, mode, u.ref(), f.ref(), r2.ref(), mgp.box(box), h, n2.ptr, None)
+        assert n1.download()[0] == n2.download()[0] > 0
+        np.testing.assert_array_equal(r2.download(), r0)
