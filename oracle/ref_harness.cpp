@@ -360,6 +360,31 @@ int run_op(const Params& p, const std::string& in, const std::string& out) {
     return 0;
 }
 
+// The reference's VTK writers (include/array.h:148-269) on a fixed field:
+// v(i,j,k) = sin(0.1 + 0.37 i + 1.1 j + 2.3 k), shape nx x ny x nz, one halo
+// layer east/north/top filled with -(index+1) so print_halo shows it.
+int run_vtk(const Params& p, const std::string& out) {
+    Settings s;
+    s.origin[0] = p.f("ox", 0.5);
+    s.origin[1] = p.f("oy", -1.0);
+    s.origin[2] = p.f("oz", 2.0);
+    s.h = p.f("h", 0.25);
+    const uint_t nx = (uint_t)p.i("nx", 3), ny = (uint_t)p.i("ny", 4), nz = (uint_t)p.i("nz", 5);
+    Array<double> a(nx, ny, nz);
+    for (uint_t i = 0; i < nx; i++)
+        for (uint_t j = 0; j < ny; j++)
+            for (uint_t k = 0; k < nz; k++) a.at[a.idx(i, j, k)] = std::sin(0.1 + 0.37 * i + 1.1 * j + 2.3 * k);
+    a.print(s, (out + "/u.vtk").c_str());
+    Halo hl(1, 0, 1, 0, 1, 0);
+    Array<double> b(nx, ny, nz, hl);
+    for (uint_t t = 0; t < b.size; t++) b.at[t] = -(double)(t + 1);
+    for (uint_t i = 0; i < nx; i++)
+        for (uint_t j = 0; j < ny; j++)
+            for (uint_t k = 0; k < nz; k++) b.at[b.idx(i, j, k)] = std::sin(0.1 + 0.37 * i + 1.1 * j + 2.3 * k);
+    b.print_halo(s, (out + "/u_halo.vtk").c_str());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -373,6 +398,7 @@ int main(int argc, char** argv) {
     const bool dbl = p.s("dtype", "f64") == "f64";
     const std::string mode = p.s("mode", "solve");
     if (mode == "op") return dbl ? run_op<double>(p, in, out) : run_op<float>(p, in, out);
+    if (mode == "vtk") return run_vtk(p, out);
     const bool fw = p.s("restriction", "inj") == "fw";
     const bool gs = p.s("relaxation", "gs") == "gs";
     if (dbl) {

@@ -168,6 +168,17 @@ def op_bc_restrict_case(name, dims, bc, restriction, seed):
     print("%-28s bc_restrict %s" % (name, restriction))
 
 
+def vtk_case(name, nx, ny, nz, origin, h):
+    """The reference's VTK writers (include/array.h:148-269) on a fixed field,
+    the files kept byte for byte."""
+    params = dict(mode="vtk", nx=nx, ny=ny, nz=nz, ox=repr(float(origin[0])), oy=repr(float(origin[1])),
+                  oz=repr(float(origin[2])), h=repr(float(h)))
+    out = run(params)
+    np.savez_compressed(os.path.join(HERE, name + ".npz"), params=json.dumps(params),
+                        vtk=np.frombuffer(out["u"], np.uint8), vtk_halo=np.frombuffer(out["u_halo"], np.uint8))
+    print("%-28s %d + %d bytes" % (name, len(out["u"]), len(out["u_halo"])))
+
+
 def main():
     if not os.path.exists(HARNESS):
         sys.exit("build the reference harness first: make -C oracle ref")
@@ -227,6 +238,7 @@ def main():
                      [0, 0, 1, 1, 0, 1], 10)
     op_bc_restrict_case("op_bc_restrict_fw_17", (17, 17, 17), "NNNNNN", "fw", 11)
     op_bc_restrict_case("op_bc_restrict_inj_17x9x9", (17, 9, 9), "DNNNND", "inj", 12)
+    vtk_case("fmt_vtk_3x4x5", 3, 4, 5, (0.5, -1.0, 2.0), 0.25)
 
 
 if __name__ == "__main__":
