@@ -112,14 +112,27 @@ int mgp_rbgs_half_f32(int mode, const mgp_array* u, const mgp_array* f, const in
                       double h, int colour, void* stream);
 /* One full red-then-black sweep in one pass, out of place: u_out = GS(u_in).
  * Neumann ghosts are formed on the fly from `faces`, as if refreshed before
- * each half-sweep (include/relaxation.h:35-52).  Points outside the box are not
- * written (u_out must already hold the boundary values).  Requires a
+ * each half-sweep (include/relaxation.h:35-52).  The shape points outside the
+ * box are copied from u_in, so u_out holds what the in-place half-sweeps
+ * leave in the shape; the halo of u_out is not written (ghosts are formed
+ * again before every use).  Requires a
  * checkerboard colouring: always in consistent mode; in reference mode only
  * when stride[0] and stride[1] are odd (MGP_ERR_ARG otherwise). */
 int mgp_rbgs_sweep_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 int mgp_rbgs_sweep_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+/* Two such sweeps in one pass: u_out = GS(GS(u_in)), bit for bit the result of
+ * mgp_rbgs_sweep_* twice through a second buffer (the GaussSeidel smoother's
+ * pairs of sweeps, include/gaussseidel.h:28-94 run twice per point; no
+ * single counterpart in the reference), shape points outside the box copied
+ * as above.  u_out must be a different buffer; MGP_ERR_ARG also where the
+ * level has fewer than
+ * 3 x-planes or a plane of 1 GiB or more (run single sweeps there). */
+int mgp_rbgs_sweep2_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+int mgp_rbgs_sweep2_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 /* nsweeps such sweeps in one launch, for a level small enough for one
  * workgroup's LDS ((nx+2)(ny+2)(nz+2) + nx ny nz elements <= 64 KiB;
  * MGP_ERR_ARG otherwise): the result of the last sweep goes to u_out, which may
@@ -404,7 +417,11 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * tile, default 4), "sweep.waves" (4, 8 or 16 waves per workgroup, kernel 1,
  * default 8), "sweep.rows" (8, 12 or 16 register rows per wave tile, kernel 2;
  * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
- * loads in flight per wave, kernel 2, default 1), "residual.kernel" (1:
+ * loads in flight per wave, kernel 2, default 1), "sweep.pair" (1: the
+ * GaussSeidel smoother runs its sweeps in pairs through mgp_rbgs_sweep2_*,
+ * default; 0: one pass per sweep), "sweep2.rows" (register rows per two-sweep
+ * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16 for FP64 and 20
+ * for FP32, default), "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
  * "residual.rows" (8 or 16 register rows per residual tile, default 16),
  * "prolong.kernel" (1: per-point kernel; 2: wave tiles marching along x for

@@ -286,6 +286,16 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
         const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
         uo.at(i, j, k) = su[pc[m]];
     }
+    // out of place: the shape points outside the box as loaded, so u_out is a
+    // complete image of the in-place result (the boundary points' own history,
+    // signs of zero included, follows whichever buffer ends up current)
+    if (uo.p != ui.p) {
+        for (int t = tid; t < n0 * n1 * n2; t += nt) {
+            const int i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
+            if (i >= bx.x0 && i < bx.x1 && j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1) continue;
+            uo.at(i, j, k) = su[L(i, j, k)];
+        }
+    }
 }
 
 // include/jacobi.h:39-82: u = J(v), out of place; the reference writes
@@ -574,6 +584,32 @@ static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const 
         return MGP_ERR_ARG;
     if (box_empty(bx)) return MGP_OK;
     return march_sweep<T>(mode, ui, uo, f, faces, b, omega, march_xchunk(b), stream);
+}
+
+// two sweeps in one pass, u_out = GS(GS(u_in)), u_out a different buffer
+template <class T>
+static int rbgs_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,
+                       const mgp_faces* faces, const int64_t b[6], double omega, void* stream) {
+    if (!valid_array(ui) || !valid_array(uo) || !valid_array(f) || !faces || !same_shape(ui, uo) ||
+        !same_shape(ui, f) || ui->data == uo->data || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
+        return MGP_ERR_ARG;
+    if (!checkerboard(mode, ui) || lin0_of(ui) != lin0_of(uo) || ui->stride[0] != uo->stride[0] ||
+        ui->stride[1] != uo->stride[1])
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < 0 || b[2 * d + 1] > ui->shape[d]) return MGP_ERR_ARG;
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == ui->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == ui->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == ui->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
+    if (box_empty(bx)) return MGP_OK;
+    return march_sweep2<T>(mode, ui, uo, f, faces, b, omega, stream);
 }
 
 // LDS bytes the multi-sweep kernel needs for a level, 0 when it does not fit
@@ -914,6 +950,14 @@ int mgp_rbgs_sweep_f64(int m, const mgp_array* ui, const mgp_array* uo, const mg
 int mgp_rbgs_sweep_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                        const int64_t b[6], double w, void* s) {
     return rbgs_sweep<float>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep2_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                        const int64_t b[6], double w, void* s) {
+    return rbgs_sweep2<double>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep2_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                        const int64_t b[6], double w, void* s) {
+    return rbgs_sweep2<float>(m, ui, uo, f, F, b, w, s);
 }
 int mgp_rbgs_sweeps_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                         const int64_t b[6], double w, int64_t n, void* s) {

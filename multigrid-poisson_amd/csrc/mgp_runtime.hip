@@ -125,6 +125,10 @@ int mgp_tuning_set(const char* key, int64_t value) {
     else if (!strcmp(key, "sweep.rows") && (value == 0 || value == 8 || value == 12 || value == 16))
         mgp::g_sweep_rows = value;
     else if (!strcmp(key, "sweep.depth") && (value == 1 || value == 2)) mgp::g_sweep_depth = value;
+    else if (!strcmp(key, "sweep.pair") && (value == 0 || value == 1)) mgp::g_sweep_pair = value;
+    else if (!strcmp(key, "sweep2.rows") &&
+             (value == 0 || value == 12 || value == 16 || value == 20 || value == 24))
+        mgp::g_sweep2_rows = value;
     else if (!strcmp(key, "residual.kernel") && (value == 1 || value == 2)) mgp::g_resid_kernel = value;
     else if (!strcmp(key, "residual.rows") && (value == 8 || value == 16)) mgp::g_resid_rows = value;
     else if (!strcmp(key, "prolong.kernel") && (value == 1 || value == 2)) mgp::g_prolong_kernel = value;
@@ -140,6 +144,8 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "sweep.kernel")) *value = mgp::g_sweep_kernel;
     else if (!strcmp(key, "sweep.rows")) *value = mgp::g_sweep_rows;
     else if (!strcmp(key, "sweep.depth")) *value = mgp::g_sweep_depth;
+    else if (!strcmp(key, "sweep.pair")) *value = mgp::g_sweep_pair;
+    else if (!strcmp(key, "sweep2.rows")) *value = mgp::g_sweep2_rows;
     else if (!strcmp(key, "residual.kernel")) *value = mgp::g_resid_kernel;
     else if (!strcmp(key, "residual.rows")) *value = mgp::g_resid_rows;
     else if (!strcmp(key, "prolong.kernel")) *value = mgp::g_prolong_kernel;

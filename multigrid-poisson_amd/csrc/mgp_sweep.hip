@@ -28,6 +28,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <type_traits>
 
 #include "mgp.h"
 #include "mgp_device.h"
@@ -56,6 +57,58 @@ struct MarchArgs {
     int ytiles, ztiles, ntiles, ntiles_pad;
     int64_t xchunk;
 };
+
+// The shape points outside the box, copied from ui to uo by extra blocks of a
+// sweep launch (the blocks after the tiles): an out-of-place pass then leaves
+// a complete image of what the in-place half-sweeps leave, whichever of the
+// two buffers ends up current (the boundary points' own history of updates,
+// signs of zero included, stays with the solution).  The halo is not copied:
+// ghost values are formed again before every use.
+constexpr int COPY_PER_THREAD = 8;
+
+template <class T>
+inline int64_t complement_points(const View<T>& u, const int64_t b[6]) {
+    const int64_t BX = b[1] - b[0], BY = b[3] - b[2];
+    return (b[0] + u.n0 - b[1]) * u.n1 * u.n2 + BX * (b[2] + u.n1 - b[3]) * u.n2 + BX * BY * (b[4] + u.n2 - b[5]);
+}
+template <class T>
+inline int complement_blocks(const View<T>& u, const int64_t b[6], int threads) {
+    const int64_t per = (int64_t)threads * COPY_PER_THREAD;
+    return (int)((complement_points(u, b) + per - 1) / per);
+}
+
+// points [blk * blockDim * COPY_PER_THREAD, ...) of the complement, in the
+// order: x outside the box (whole planes), then y outside (box planes), then
+// z outside (box rows); consecutive threads take consecutive z where it runs
+template <class T>
+__device__ void copy_complement(const MarchArgs<T>& A, int blk) {
+    const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2;
+    const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
+    const int64_t xo = A.bx0 + n0 - A.bx1, yo = A.by0 + n1 - A.by1, zo = A.bz0 + n2 - A.bz1;
+    const int64_t c1 = xo * n1 * n2, c2 = BX * yo * n2, c3 = BX * BY * zo;
+    const int64_t per = (int64_t)blockDim.x * COPY_PER_THREAD;
+    const int64_t end = min(c1 + c2 + c3, (int64_t)(blk + 1) * per);
+    for (int64_t t = (int64_t)blk * per + threadIdx.x; t < end; t += blockDim.x) {
+        int64_t x, y, z;
+        if (t < c1) {
+            const int64_t a = t / (n1 * n2), r = t % (n1 * n2);
+            x = a < A.bx0 ? a : A.bx1 + (a - A.bx0);
+            y = r / n2;
+            z = r % n2;
+        } else if (t < c1 + c2) {
+            const int64_t u = t - c1, a = u / (yo * n2), r = u % (yo * n2), c = r / n2;
+            x = A.bx0 + a;
+            y = c < A.by0 ? c : A.by1 + (c - A.by0);
+            z = r % n2;
+        } else {
+            const int64_t u = t - c1 - c2, a = u / (BY * zo), r = u % (BY * zo), c = r % zo;
+            x = A.bx0 + a;
+            y = A.by0 + r / zo;
+            z = c < A.bz0 ? c : A.bz1 + (c - A.bz0);
+        }
+        A.uo.at(x, y, z) = A.ui.at(x, y, z);
+    }
+}
 
 // element at a 32-bit byte offset from a wave-uniform base: lets the compiler
 // use the scalar-base + 32-bit vector-offset addressing form
@@ -121,6 +174,10 @@ __global__ void __launch_bounds__(64 * (RH / NQ)) k_rbgs_march(MarchArgs<T> A) {
     // XCD-aware tile order: workgroups are dealt to the 8 XCDs round-robin, so
     // give each XCD a contiguous run of tiles (neighbours share halo rows in
     // that XCD's L2)
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (tl >= A.ntiles) return;
@@ -588,6 +645,10 @@ __device__ __forceinline__ void wave_march(const MarchArgs<T>& A, int tl) {
 
 template <class T, int MODE, int R, int D>
 __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (tl >= A.ntiles) return;
@@ -599,6 +660,211 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
                       (z0 + WZ + 1 >= A.ui.n2 && A.g[MGP_TOP]);
     if (edge) wave_march<T, MODE, R, D, true>(A, tl);
     else wave_march<T, MODE, R, D, false>(A, tl);
+}
+
+
+// ---------------------------------------------------------------------------
+// Two sweeps in one pass (mgp_rbgs_sweep2_*): the wave march above with four
+// half-sweep stages in flight instead of two.  At plane step i the wave updates
+//   stage 0: sweep 1 red   of plane i,
+//   stage 1: sweep 1 black of plane i-1,
+//   stage 2: sweep 2 red   of plane i-2,
+//   stage 3: sweep 2 black of plane i-3, then writes plane i-3,
+// in that order, so each stage's x-neighbours hold exactly the values the
+// previous half-sweep left there: the planes on both sides of a stage's plane
+// have been through the previous stage and not yet through this one.  In y and
+// z every stage is valid one point further inside the tile than the stage
+// before it: register rows t = 0 .. NR-1 (y = y0 - 4 + t) and lanes 0 .. 63
+// (z = z0 - 4 + lane) shrink to rows 4 .. NR-5 and lanes 4 .. 59 after stage
+// 3, which is what the tile writes (NR - 8 rows x 56 columns; the rings are
+// recomputed by the neighbouring tiles).  Neumann ghosts are formed before
+// every stage from the values the stage reads (the ghost refresh before each
+// half-sweep, include/gaussseidel.h:28-94 with include/neumann.h:108-124):
+// in-plane ghosts from the rows / lanes two apart, the x-ghost planes -1 and
+// n0 from the planes two apart.  Per point the arithmetic is the single
+// sweep's, so the result equals two sweeps bit for bit.
+constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
+
+template <class T, int MODE, int NR, bool EDGE>
+__device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
+    constexpr int RO = NR - 8;  // rows written per tile
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * RO;
+    const int64_t z0 = A.bz0 + (int64_t)tz * WZ2;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+    const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2, s0 = A.ui.s0, s1 = A.ui.s1;
+    const T two_h = A.two_h;
+    const uint32_t plane_bytes = (uint32_t)(((n1 - 1) * s1 + n2) * (int64_t)sizeof(T));
+
+    const int lane = threadIdx.x;
+    const int64_t z = z0 - 4 + lane;
+    const bool z_in = z >= A.bz0 && z < A.bz1;
+    const bool z_shape = z >= 0 && z < n2;
+    const int64_t zc = min(max(z, (int64_t)0), n2 - 1);
+    const uint32_t zoff = (uint32_t)(zc * (int64_t)sizeof(T));
+    const bool lane_core = z_in && lane >= 4 && lane <= 59;
+    const uint32_t zoff_st = lane_core ? zoff : BUF_OOB;
+    // colour as in the single sweep (rows are y0 - 4 + t: same parity as y0 + t)
+    const int cbase = (int)(((MODE == MGP_MODE_REFERENCE ? A.lin0 : 0) + z + y0) & 1);
+
+    uint32_t roff[NR];
+    bool ybox[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int64_t y = y0 - 4 + t;
+        roff[t] = (uint32_t)(min(max(y, (int64_t)0), n1 - 1) * s1 * (int64_t)sizeof(T));
+        ybox[t] = y >= A.by0 && y < A.by1;
+    }
+
+    const T* gW = A.g[MGP_WEST];
+    const T* gE = A.g[MGP_EAST];
+    const T* gS = A.g[MGP_SOUTH];
+    const T* gN = A.g[MGP_NORTH];
+    const T* gB = A.g[MGP_BOTTOM];
+    const T* gT = A.g[MGP_TOP];
+    const bool y_ghosts = (y0 - 4 < 0 && gS) || (y0 + NR - 5 >= n1 && gN);
+    const bool z_ghosts = (z0 - 4 < 0 && gB) || (z0 + 59 >= n2 && gT);
+
+    auto load_plane = [&](int64_t j, T (&P)[NR]) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.ui.p + min(max(j, (int64_t)0), n0 - 1) * s0, plane_bytes);
+#pragma unroll
+        for (int t = 0; t < NR; t++) P[t] = bload<T>(r, zoff, roff[t]);
+    };
+    auto load_f = [&](int64_t j, T (&F)[NR]) {
+        const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.f.p + min(max(j, (int64_t)0), n0 - 1) * s0, plane_bytes);
+#pragma unroll
+        for (int t = 1; t <= NR - 2; t++) F[t] = bload<T>(r, zoff, roff[t]);
+    };
+
+    // stage S at plane p (S even: red, odd: black) with west / centre / east
+    // planes W, C, E and f of plane p; rows S+1 .. NR-2-S are updated, rows
+    // S .. NR-1-S read
+    auto stage = [&](auto Sc, int64_t p, T (&W)[NR], T (&C)[NR], T (&E)[NR], const T (&F)[NR]) {
+        constexpr int S = decltype(Sc)::value;
+        // x-ghost planes from the plane two apart (face layout (1, ny, nz))
+        if (p == 0 && gW) {
+#pragma unroll
+            for (int t = S + 1; t <= NR - 2 - S; t++) {
+                const int64_t yc = min(max(y0 - 4 + t, (int64_t)0), n1 - 1);
+                W[t] = (T)-1.0 * two_h * gW[yc * n2 + zc] + E[t];
+            }
+        }
+        if (p == n0 - 1 && gE) {
+#pragma unroll
+            for (int t = S + 1; t <= NR - 2 - S; t++) {
+                const int64_t yc = min(max(y0 - 4 + t, (int64_t)0), n1 - 1);
+                E[t] = (T)1.0 * two_h * gE[yc * n2 + zc] + W[t];
+            }
+        }
+        // in-plane ghosts of plane p among the rows the stage reads
+        if (EDGE && y_ghosts) {
+#pragma unroll
+            for (int t = S; t <= NR - 1 - S; t++) {
+                const int64_t y = y0 - 4 + t;
+                if (t + 2 <= NR - 1 - S && y == -1 && gS) {
+                    const T v = (T)-1.0 * two_h * gS[p * n2 + zc] + C[t + 2];  // face (nx, 1, nz)
+                    C[t] = z_shape ? v : C[t];
+                } else if (t - 2 >= S && y == n1 && gN) {
+                    const T v = (T)1.0 * two_h * gN[p * n2 + zc] + C[t - 2];
+                    C[t] = z_shape ? v : C[t];
+                }
+            }
+        }
+        if (EDGE && z_ghosts) {
+#pragma unroll
+            for (int t = S; t <= NR - 1 - S; t++) {
+                const int64_t y = y0 - 4 + t;
+                if (y < 0 || y >= n1) continue;
+                if (gB) {
+                    const T m = lane_next(lane_next(C[t]));
+                    const T v = (T)-1.0 * two_h * gB[p * n1 + y] + m;  // face (nx, ny, 1)
+                    C[t] = z == -1 ? v : C[t];
+                }
+                if (gT) {
+                    const T m = lane_prev(lane_prev(C[t]));
+                    const T v = (T)1.0 * two_h * gT[p * n1 + y] + m;
+                    C[t] = z == n2 ? v : C[t];
+                }
+            }
+        }
+        const bool lane_ok = z_in && lane >= S + 1 && lane <= 62 - S;
+        const int want = S & 1;  // 0: red, 1: black
+        const int par = (int)(p & 1);
+#pragma unroll
+        for (int t = S + 1; t <= NR - 2 - S; t++) {
+            const bool upd = lane_ok && (!EDGE || ybox[t]) && ((cbase ^ par ^ (t & 1)) == want);
+            const T v = update<T, MODE>(C[t], W[t], E[t], C[t - 1], C[t + 1], lane_prev(C[t]), lane_next(C[t]),
+                                        F[t], A.one_omega, A.omega_sixth, A.hsq);
+            C[t] = upd ? v : C[t];
+        }
+    };
+
+    // x window: planes i-4 .. i+1 and the plane being loaded (i+2); f of
+    // planes i-3 .. i and the f plane being loaded (i+1)
+    T Pm4[NR], Pm3[NR], Pm2[NR], Pm1[NR], P0[NR], Pp1[NR], PL[NR];
+    T Fm3[NR], Fm2[NR], Fm1[NR], F0[NR], FL[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) Pm4[t] = Pm3[t] = Pm2[t] = Fm3[t] = Fm2[t] = Fm1[t] = (T)0;
+    const int64_t i0 = xa - 3;
+    load_plane(i0 - 1, Pm1);
+    load_plane(i0, P0);
+    load_plane(i0 + 1, Pp1);
+    load_f(i0, F0);
+    using S0 = std::integral_constant<int, 0>;
+    using S1 = std::integral_constant<int, 1>;
+    using S2 = std::integral_constant<int, 2>;
+    using S3 = std::integral_constant<int, 3>;
+    for (int64_t i = i0; i <= xb + 2; i++) {
+        load_plane(i + 2, PL);
+        load_f(i + 1, FL);
+        // each stage from the first plane the chunk's output depends on
+        if (i >= A.bx0 && i < A.bx1) stage(S0{}, i, Pm1, P0, Pp1, F0);
+        if (i - 1 >= max(A.bx0, xa - 2) && i - 1 < A.bx1) stage(S1{}, i - 1, Pm2, Pm1, P0, Fm1);
+        if (i - 2 >= max(A.bx0, xa - 1) && i - 2 < A.bx1) stage(S2{}, i - 2, Pm3, Pm2, Pm1, Fm2);
+        const int64_t p = i - 3;
+        if (p >= xa && p < xb) {
+            stage(S3{}, p, Pm4, Pm3, Pm2, Fm3);
+            const __amdgpu_buffer_rsrc_t ro = plane_rsrc(A.uo.p + p * s0, plane_bytes);
+#pragma unroll
+            for (int t = 4; t <= NR - 5; t++) bstore(Pm3[t], ro, !EDGE || ybox[t] ? zoff_st : BUF_OOB, roff[t]);
+        }
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            Pm4[t] = Pm3[t];
+            Pm3[t] = Pm2[t];
+            Pm2[t] = Pm1[t];
+            Pm1[t] = P0[t];
+            P0[t] = Pp1[t];
+            Pp1[t] = PL[t];
+            Fm3[t] = Fm2[t];
+            Fm2[t] = Fm1[t];
+            Fm1[t] = F0[t];
+            F0[t] = FL[t];
+        }
+    }
+}
+
+template <class T, int MODE, int NR>
+__global__ void __launch_bounds__(64) k_rbgs2_wave(MarchArgs<T> A) {
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int ty = (tl / A.ztiles) % A.ytiles, tz = tl % A.ztiles;
+    const int64_t y0 = A.by0 + (int64_t)ty * (NR - 8), z0 = A.bz0 + (int64_t)tz * WZ2;
+    // rows y0-3 .. y0+NR-6 (the ones stage 0 updates) inside the box, and no
+    // ghost positions among the tile's rows and lanes
+    const bool edge = y0 - 3 < A.by0 || y0 + NR - 6 >= A.by1 || (y0 - 4 < 0 && A.g[MGP_SOUTH]) ||
+                      (y0 + NR - 5 >= A.ui.n1 && A.g[MGP_NORTH]) || (z0 - 4 < 0 && A.g[MGP_BOTTOM]) ||
+                      (z0 + 59 >= A.ui.n2 && A.g[MGP_TOP]);
+    if (edge) wave_march2<T, MODE, NR, true>(A, tl);
+    else wave_march2<T, MODE, NR, false>(A, tl);
 }
 
 
@@ -1057,7 +1323,7 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
         A.xchunk = (nx + chunks - 1) / chunks;
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
         A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
-        const dim3 g(A.ntiles_pad);
+        const dim3 g(A.ntiles_pad + complement_blocks(A.ui, b, 64));
 #define MGP_WAVE_LAUNCH(RR, DD)                                                                          \
     if (ref) hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_REFERENCE, RR, DD>), g, dim3(64), 0, st, A); \
     else hipLaunchKernelGGL((k_rbgs_wave<T, MGP_MODE_CONSISTENT, RR, DD>), g, dim3(64), 0, st, A);
@@ -1069,7 +1335,8 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
         const hipError_t e = hipGetLastError();
         return e == hipSuccess ? MGP_OK : (int)e;
     }
-    const dim3 g(A.ntiles_pad);
+    const int threads = g_sweep_waves == 4 ? 256 : g_sweep_waves == 16 ? 1024 : 512;
+    const dim3 g(A.ntiles_pad + complement_blocks(A.ui, b, threads));
     switch (g_sweep_waves) {
     case 4:
         if (ref) hipLaunchKernelGGL((k_rbgs_march<T, MGP_MODE_REFERENCE, 4>), g, dim3(256), 0, st, A);
@@ -1282,5 +1549,58 @@ template int march_sweep<double>(int, const mgp_array*, const mgp_array*, const 
                                  const int64_t[6], double, int64_t, void*);
 template int march_sweep<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
                                 const int64_t[6], double, int64_t, void*);
+
+int64_t g_sweep_pair = 1;   // relax_level runs sweeps in pairs through the two-sweep pass
+int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, FP32 16, 20 or 24; 0: by type
+
+template <class T>
+int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                 const int64_t b[6], double omega, void* stream) {
+    MarchArgs<T> A;
+    A.ui = view_of<T>(*ui);
+    A.uo = view_of<T>(*uo);
+    A.f = view_of<T>(*f);
+    // planes addressed through buffer descriptors with a 32-bit range
+    const int64_t plane_bytes = ((A.ui.n1 - 1) * A.ui.s1 + A.ui.n2) * (int64_t)sizeof(T);
+    if (A.ui.n0 < 3 || plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++) A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const T*>(faces->g[q]) : nullptr;
+    A.two_h = (T)(2.0 * faces->h);  // include/neumann.h:108
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.lin0 = ui->halo[MGP_WEST] * ui->stride[0] + ui->halo[MGP_SOUTH] * ui->stride[1] + ui->halo[MGP_BOTTOM];
+    const T om = (T)omega;  // include/gaussseidel.h:50-52
+    A.one_omega = (T)(1.0 - om);
+    A.omega_sixth = (T)(om / 6.0);
+    A.hsq = (T)(faces->h * faces->h);
+    int rows = (int)g_sweep2_rows;
+    if (sizeof(T) == 8 && rows != 12 && rows != 16) rows = 16;
+    if (sizeof(T) == 4 && rows != 16 && rows != 20 && rows != 24) rows = 20;
+    A.ytiles = (int)((b[3] - b[2] + rows - 9) / (rows - 8));
+    A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
+    const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+    const int64_t nx = b[1] - b[0];
+    int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+    A.xchunk = (nx + chunks - 1) / chunks;
+    A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
+    A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    const dim3 g(A.ntiles_pad + complement_blocks(A.ui, b, 64));
+    const hipStream_t st = (hipStream_t)stream;
+    const bool ref = mode == MGP_MODE_REFERENCE;
+#define MGP_WAVE2_LAUNCH(RR)                                                                            \
+    if (ref) hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_REFERENCE, RR>), g, dim3(64), 0, st, A); \
+    else hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_CONSISTENT, RR>), g, dim3(64), 0, st, A);
+    if constexpr (sizeof(T) == 8) {
+        if (rows == 12) { MGP_WAVE2_LAUNCH(12) } else { MGP_WAVE2_LAUNCH(16) }
+    } else {
+        if (rows == 16) { MGP_WAVE2_LAUNCH(16) } else if (rows == 24) { MGP_WAVE2_LAUNCH(24) } else { MGP_WAVE2_LAUNCH(20) }
+    }
+#undef MGP_WAVE2_LAUNCH
+    const hipError_t e = hipGetLastError();
+    return e == hipSuccess ? MGP_OK : (int)e;
+}
+template int march_sweep2<double>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
+                                  const int64_t[6], double, void*);
+template int march_sweep2<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
+                                 const int64_t[6], double, void*);
 
 }  // namespace mgp

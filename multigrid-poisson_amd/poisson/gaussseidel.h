@@ -5,7 +5,9 @@
 // REFERENCE mode when both outer strides are odd) and Settings::fused_sweeps is
 // on, relax_level runs each red+black pair as one out-of-place pass
 // (mgp_rbgs_sweep) that forms Neumann ghosts on the fly; the values are
-// identical to the two half-sweeps with boundary refreshes in between.
+// identical to the two half-sweeps with boundary refreshes in between; pairs
+// of such sweeps run as one pass (mgp_rbgs_sweep2) unless the "sweep.pair"
+// tuning key is 0.
 #ifndef POISSON_GAUSS_SEIDEL
 #define POISSON_GAUSS_SEIDEL
 
@@ -57,7 +59,22 @@ public:
                 return;
             }
         }
-        for (int_t s = 0; s < nsmooth; s++) {
+        int_t s = 0;
+        int64_t pair = 1;
+        if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
+        // pairs of sweeps in one pass each (MGP_ERR_ARG: a shape the two-sweep
+        // pass does not take; single sweeps then)
+        for (; pair && s + 2 <= nsmooth; s += 2) {
+            mgp_array ui = d.u->desc();
+            mgp_array uo = ui;
+            uo.data = d.u_alt;
+            const int rc = mgpx::Ops<T>::rbgs_sweep2(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega,
+                                                     mgpx::stream());
+            if (rc == MGP_ERR_ARG) break;
+            mgpx::check(rc, "GaussSeidel sweep pair");
+            d.swap_u_alt();
+        }
+        for (; s < nsmooth; s++) {
             mgp_array ui = d.u->desc();
             mgp_array uo = ui;
             uo.data = d.u_alt;

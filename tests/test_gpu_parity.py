@@ -5,6 +5,7 @@ bit for bit (FP64 and FP32); consistent mode must reproduce the oracle bit for
 bit.  Every case runs with graph replay on and off and with fused sweeps on
 and off.
 """
+import contextlib
 import ctypes
 import sys
 import zlib
@@ -509,6 +510,168 @@ def test_lds_multi_sweep_rejects_large_levels():
     rc = mgp.lib().mgp_rbgs_sweeps_f64(mgp.CONSISTENT, u.ref(), u.ref(), u.ref(), ctypes.byref(F),
                                        mgp.box([1, 64, 1, 64, 1, 32]), 1.0, 3, None)
     assert rc == mgp.ERR_ARG
+
+
+SWEEP2_SHAPES = SWEEP_SHAPES + [((3, 40, 200), "NNDDNN"), ((21, 3, 3), "NNNNNN"), ((12, 57, 113), "DDNNDD")]
+# (dtype, sweep2.rows, sweep.target_wgs): several x-chunks per tile, and one
+SWEEP2_CASES = [("f64", 16, 16384), ("f64", 12, 16384), ("f64", 16, 1), ("f32", 20, 16384), ("f32", 16, 16384),
+                ("f32", 24, 1)]
+
+
+@contextlib.contextmanager
+def tuned(**kv):
+    """Tuning keys (dots written as __) set for the block, restored after."""
+    L = mgp.lib()
+    old = {}
+    for k, v in kv.items():
+        key = k.replace("__", ".").encode()
+        cur = ctypes.c_int64()
+        mgp.check(L.mgp_tuning_get(key, ctypes.byref(cur)))
+        old[key] = cur.value
+        mgp.check(L.mgp_tuning_set(key, v))
+    try:
+        yield
+    finally:
+        for key, v in old.items():
+            mgp.check(L.mgp_tuning_set(key, v))
+
+
+def shape_slices(shape, halo):
+    """Index of the shape inside a halo-padded array (halo order N S E W T B)."""
+    hN, hS, hE, hW, hT, hB = halo
+    return (slice(hW, hW + shape[0]), slice(hS, hS + shape[1]), slice(hB, hB + shape[2]))
+
+
+def random_level(shape, bc, mode, dtype, seed):
+    """Random u (halo included), f and Neumann faces of a level, its faces
+    struct and relaxation box; None where the reference colouring is not a
+    checkerboard."""
+    halo = [1 if c == "N" else 0 for c in bc]
+    if mode == mgp.REFERENCE:
+        p = mgp.padded_shape(list(shape), halo)
+        if not ((p[1] * p[2]) % 2 and p[2] % 2):
+            return None
+    rng = np.random.default_rng(zlib.crc32(repr(seed).encode()))
+    u0 = rng.standard_normal(mgp.padded_shape(list(shape), halo))
+    f = mgp.DeviceArray(shape, halo, dtype=dtype).upload(rng.standard_normal(u0.shape))
+    faces = {}
+    F = mgp.Faces()
+    for q in range(6):
+        n = [shape[0], shape[1], shape[2]]
+        n[{0: 1, 1: 1, 2: 0, 3: 0, 4: 2, 5: 2}[q]] = 1
+        faces[q] = mgp.DeviceBuffer(rng.standard_normal(int(np.prod(n))).astype(dtype))
+        F.type[q] = mgp.NEUMANN if bc[q] == "N" else mgp.DIRICHLET
+        F.g[q] = faces[q].ptr if bc[q] == "N" else None
+    F.h = 1.0 / (shape[0] - 1)
+    box = [int(bc[3] == "D"), shape[0] - int(bc[2] == "D"), int(bc[1] == "D"), shape[1] - int(bc[0] == "D"),
+           int(bc[5] == "D"), shape[2] - int(bc[4] == "D")]
+    return halo, u0, f, faces, F, box
+
+
+@pytest.mark.parametrize("case", SWEEP2_CASES, ids=["%s_r%d_w%d" % c for c in SWEEP2_CASES])
+@pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
+@pytest.mark.parametrize("shape,bc", SWEEP2_SHAPES, ids=[str(i) for i in range(len(SWEEP2_SHAPES))])
+def test_two_sweep_pass_equals_two_sweeps(shape, bc, mode, case):
+    """mgp_rbgs_sweep2 (two red-black sweeps in one x-march, four half-sweep
+    stages in flight) == two mgp_rbgs_sweep passes, bit for bit, including the
+    points and halo it must not write, with partial tiles in y and z, several
+    and single x-chunks, and every face mix of Neumann ghosts."""
+    sfx, rows, wgs = case
+    dtype = np.float64 if sfx == "f64" else np.float32
+    say("case sweep2", shape, bc, mode, case)
+    lv = random_level(shape, bc, mode, dtype, (shape, bc, mode, "sweep2"))
+    if lv is None:
+        pytest.skip("reference colouring is not a checkerboard for this halo layout")
+    halo, u0, f, faces, F, box = lv
+    a = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    b = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    for _ in range(2):
+        mgp.call("mgp_rbgs_sweep_" + sfx, mode, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93, None)
+        a, b = b, a
+    want = a.download()
+    # the output buffer starts with other values: its shape must come out
+    # whole (box points swept, the rest copied from the input), its halo untouched
+    other = np.random.default_rng(7).standard_normal(u0.shape)
+    c = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    d = mgp.DeviceArray(shape, halo, dtype=dtype).upload(other)
+    with tuned(sweep2__rows=rows, sweep__target_wgs=wgs):
+        mgp.call("mgp_rbgs_sweep2_" + sfx, mode, c.ref(), d.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
+                 None)
+    got = d.download()
+    inner = shape_slices(shape, halo)
+    np.testing.assert_array_equal(got[inner], want[inner])
+    outside = np.ones(got.shape, bool)
+    outside[inner] = False
+    np.testing.assert_array_equal(got[outside], other.astype(dtype)[outside])
+    np.testing.assert_array_equal(c.download(), u0.astype(dtype))  # the input is not written
+
+
+@pytest.mark.parametrize("sweep_kernel", [(1, 16, 1), (2, 12, 1)], ids=["k1", "k2"], indirect=True)
+@pytest.mark.parametrize("shape,bc", [((9, 17, 65), "DDDDDD"), ((17, 15, 129), "DNNDDN"), ((33, 30, 64), "NDDNND")],
+                         ids=["0", "1", "2"])
+def test_one_pass_sweep_copies_shape_outside_box(shape, bc, sweep_kernel):
+    """An out-of-place sweep leaves u_out's shape whole: box points swept,
+    the boundary points outside the box copied from u_in; u_out's halo is
+    not written."""
+    say("case sweep copy", shape, bc, sweep_kernel)
+    halo, u0, f, faces, F, box = random_level(shape, bc, mgp.CONSISTENT, np.float64, (shape, bc, "copy"))
+    other = np.random.default_rng(8).standard_normal(u0.shape)
+    a = mgp.DeviceArray(shape, halo).upload(u0)
+    b = mgp.DeviceArray(shape, halo).upload(u0)
+    c = mgp.DeviceArray(shape, halo).upload(other)
+    mgp.call("mgp_rbgs_sweep_f64", mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
+             None)
+    mgp.call("mgp_rbgs_sweep_f64", mgp.CONSISTENT, a.ref(), c.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
+             None)
+    want, got = b.download(), c.download()
+    inner = shape_slices(shape, halo)
+    np.testing.assert_array_equal(got[inner], want[inner])
+    outside = np.ones(got.shape, bool)
+    outside[inner] = False
+    np.testing.assert_array_equal(got[outside], other[outside])
+
+
+def test_two_sweep_pass_rejects_in_place_and_thin_levels():
+    shape, halo = (17, 17, 17), (0, 0, 0, 0, 0, 0)
+    u = mgp.DeviceArray(shape, halo)
+    v = mgp.DeviceArray(shape, halo)
+    F = mgp.Faces()
+    for q in range(6):
+        F.type[q] = mgp.DIRICHLET
+    F.h = 1.0 / 16
+    L = mgp.lib()
+    assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, u.ref(), u.ref(), v.ref(), ctypes.byref(F),
+                                 mgp.box([1, 16, 1, 16, 1, 16]), 1.0, None) == mgp.ERR_ARG
+    thin = (2, 17, 17)
+    a, b, f = (mgp.DeviceArray(thin, halo) for _ in range(3))
+    assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F),
+                                 mgp.box([1, 1, 1, 16, 1, 16]), 1.0, None) == mgp.OK  # empty box
+    for q in (2, 3):
+        F.type[q] = mgp.NEUMANN
+    assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F),
+                                 mgp.box([0, 2, 1, 16, 1, 16]), 1.0, None) == mgp.ERR_ARG  # Neumann without g
+
+
+@pytest.mark.parametrize("dtype", [mgp.F64, mgp.MIXED], ids=["f64", "mixed"])
+@pytest.mark.parametrize("nsmooth", [2, 3])
+def test_paired_sweeps_solve_equals_single_sweep_solve(dtype, nsmooth):
+    """A solve whose smoother runs sweep pairs through the two-sweep pass
+    (sweep.pair 1) ends with the same bits as one pass per sweep (sweep.pair 0):
+    iterations, relative residual and solution."""
+    say("case pair solve", dtype, nsmooth)
+    dims, levels = (65, 65, 33), 4
+    out = {}
+    for pair in (0, 1):
+        with tuned(sweep__pair=pair):
+            s = mgp.Solver(dims, levels, 1.0 / 64, (0, 0, 0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                           mgp.FULL_WEIGHTING, 2, dtype=dtype, maxiter=6, miniter=6, tolerance=0.0)
+            s.to_device()
+            it, rel, _ = s.solve("Fcycle", nsmooth, 1.0)
+            s.to_host()
+            out[pair] = (it, rel, s.array("u").copy())
+            s.close()
+    assert out[0][:2] == out[1][:2]
+    np.testing.assert_array_equal(out[0][2], out[1][2])
 
 
 PROLONG_SHAPES = [(1025 // 16 + 1, 33, 17), (17, 33, 65), (9, 9, 9), (33, 17, 129), (5, 3, 3)]
