@@ -120,9 +120,10 @@ def main():
     if max(rel) > args.tol:
         log("bench: WARNING: tolerance not reached")
 
-    # Dominant kernels measured live on the fine level: the fused red-black
-    # sweep (24 B per box point: read u and f, write u) and the residual with
-    # its max-norm (24 B per point: read u and f, write r).
+    # Dominant kernels measured live on the fine level: the two-sweep pass the
+    # smoother runs (24 B per box point: read u and f once, write u once, for
+    # two sweeps), the one-pass sweep (24 B per point per sweep) and the
+    # residual with its max-norm (24 B per point: read u and f, write r).
     L = mgp.lib()
     stream = mgp.vp()
     mgp.check(L.mgp_stream_create(ctypes.byref(stream)))
@@ -148,6 +149,10 @@ def main():
     sweep_ms = kernel_time_ms(
         lambda st: mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_),
                                                    ctypes.byref(F), bx, 1.0, st)), args.kernel_reps, stream)
+    pair_ms = kernel_time_ms(
+        lambda st: mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r),
+                                                    ctypes.byref(f_), ctypes.byref(F), bx, 1.0, st)),
+        args.kernel_reps, stream)
     resid_ms = kernel_time_ms(
         lambda st: mgp.check(L.mgp_residual_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(f_), ctypes.byref(r),
                                                  bx, h, scal.ptr, st)), args.kernel_reps, stream)
@@ -155,8 +160,10 @@ def main():
     sweep_bytes = 24.0 * box_pts
     resid_bytes = 24.0 * box_pts
     sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
+    pair_gbs = sweep_bytes / (pair_ms * 1e-3) / 1e9
     resid_gbs = resid_bytes / (resid_ms * 1e-3) / 1e9
-    log("bench: sweep %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" % (sweep_ms, sweep_gbs, resid_ms, resid_gbs))
+    log("bench: sweep %.2f ms (%.0f GB/s), two-sweep pass %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" %
+        (sweep_ms, sweep_gbs, pair_ms, pair_gbs, resid_ms, resid_gbs))
 
     s.close()
     mixed = guarded_leg("mixed", args.no_mixed, lambda: mixed_leg(dims, h, args))
@@ -185,10 +192,11 @@ def main():
                    "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
                    "coarsest_sweeps": args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "rbgs_sweep (fine level)", "achieved": sweep_gbs,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": sweep_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(n), "bytes_per_launch": sweep_bytes, "ms_per_launch": sweep_ms},
-        "kernels": {"residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
+        "roofline": {"bound": "hbm", "kernel": "rbgs_sweep2 (fine level, two sweeps per pass)",
+                     "achieved": pair_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pair_gbs / HBM_PEAK_GBS,
+                     "traffic": measured_traffic(n), "bytes_per_launch": sweep_bytes, "ms_per_launch": pair_ms},
+        "kernels": {"rbgs_sweep": {"ms": sweep_ms, "GB/s": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS},
+                    "residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
         "fp32_refine": mixed,
         "batched": batch,
         "serving": serve,
@@ -198,10 +206,10 @@ def main():
 
 
 def measured_traffic(n):
-    """HBM bytes per fine-level sweep launch from the rocprofv3 FETCH_SIZE /
+    """HBM bytes per fine-level two-sweep pass from the rocprofv3 FETCH_SIZE /
     WRITE_SIZE runs summarised in profiles/ (scaled by calibration kernels of
     known size), for the 1025x1025x513 workload; None for other sizes."""
-    path = os.path.join(ROOT, "profiles", "rbgs_sweep_%dx%dx%d_traffic.md" % (n, n, n // 2 + 1))
+    path = os.path.join(ROOT, "profiles", "rbgs_sweep2_%dx%dx%d_traffic.md" % (n, n, n // 2 + 1))
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -287,6 +295,43 @@ def mixed_leg(dims, h, args):
     out["cos_dirichlet_max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp64": err_fp64}
     log("bench: mixed |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" % (d_fp64, err_mixed, err_fp64))
     return out
+
+
+def traffic_probe(args):
+    """Kernels for HBM-counter runs: fill of r (8 B/point written), inf-norm of
+    f (8 B/point read), one one-pass sweep u -> r (24 B per box point) and one
+    two-sweep pass u -> r (24 B per box point)."""
+    import numpy as np
+
+    n, nz = args.n, args.n // 2 + 1
+    h = 1.0 / (n - 1)
+    s = mgp.Solver((n, n, nz), args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET | mgp.NO_GRAPHS, coarse_sweeps=args.coarse_sweeps)
+    s.to_device()
+    L = mgp.lib()
+    u, f_, r = s.device_array("u"), s.device_array("f"), s.device_array("r")
+    scal = mgp.DeviceBuffer(np.zeros(1))
+    F = mgp.Faces()
+    for q in range(6):
+        t = s.face_type(q)
+        F.type[q] = t
+        host, a, tt = mgp.vp(), mgp.Array(), ctypes.c_int()
+        mgp.check(L.mgp_solver_face(s.h, 0, q, ctypes.byref(host), ctypes.byref(a), ctypes.byref(tt)))
+        F.g[q] = a.data if t == mgp.NEUMANN else None
+    F.h = h
+    L.mgp_solver_sync(s.h)
+    box = mgp.box([0, n, 0, n, 1, nz - 1])
+    mgp.check(L.mgp_fill_f64(ctypes.byref(r), 0.0, None))
+    mgp.check(L.mgp_norm_inf_f64(ctypes.byref(f_), scal.ptr, None))
+    mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_), ctypes.byref(F),
+                                   box, 1.0, None))
+    mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_),
+                                    ctypes.byref(F), box, 1.0, None))
+    L.mgp_stream_sync(None)
+    pts = n * n * nz
+    log("traffic probe: fill %d B, norm %d B, sweep %d B, two-sweep pass %d B algorithmic" %
+        (8 * pts, 8 * pts, 24 * n * n * (nz - 2), 24 * n * n * (nz - 2)))
+    s.close()
 
 
 def guarded_leg(name, skip, fn):

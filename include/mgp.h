@@ -420,10 +420,14 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * loads in flight per wave, kernel 2, default 1), "sweep.pair" (1: the
  * GaussSeidel smoother runs its sweeps in pairs through mgp_rbgs_sweep2_*,
  * default; 0: one pass per sweep), "sweep2.rows" (register rows per two-sweep
- * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16 for FP64 and 20
- * for FP32, default), "residual.kernel" (1:
+ * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16, default),
+ * "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
  * "residual.rows" (8 or 16 register rows per residual tile, default 16),
+ * "residual_fw.rows" (4 or 6 coarse rows per wave tile of the fused residual
+ * and full-weighting restriction; 0 = 6, default), "residual_fw.depth" (1 or
+ * 2 fine planes of loads in flight per wave; 0 = 1 for FP64 and 2 for FP32,
+ * default),
  * "prolong.kernel" (1: per-point kernel; 2: wave tiles marching along x for
  * mgp_prolong_add_*, default).
  * No counterpart in the

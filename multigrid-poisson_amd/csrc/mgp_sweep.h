@@ -37,6 +37,8 @@ extern int64_t g_prolong_kernel;
 template <class T>
 int march_residual_fw(const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* cf,
                       const int64_t box[6], double h, void* stream);
+extern int64_t g_fw_rows;
+extern int64_t g_fw_depth;
 extern int64_t g_resid_kernel;
 extern int64_t g_resid_rows;
 // default x-chunk length for a box

@@ -1121,7 +1121,8 @@ struct RFArgs {
 
 constexpr int FWK = 30;  // coarse columns per wave
 
-template <class T, int RJ, bool EDGE>
+// D: fine planes of loads in flight ahead of the residual (1 or 2)
+template <class T, int RJ, int D, bool EDGE>
 __device__ __forceinline__ void resid_fw_march(const RFArgs<T>& A, int tl) {
     constexpr int NU = 2 * RJ + 3, NRr = 2 * RJ + 1;
     const int tz = tl % A.ztiles;
@@ -1201,12 +1202,16 @@ __device__ __forceinline__ void resid_fw_march(const RFArgs<T>& A, int tl) {
         (void)z_shape;
         (void)y_shape;
     };
-    T Um[NU], U0[NU], Up[NU], UL[NU], F0[NU], FL[NU], R2[NRr], R1[NRr], R0[NRr];
+    T Um[NU], U0[NU], Up[NU], Up2[NU], UL[NU], F0[NU], F1[NU], FL[NU], R2[NRr], R1[NRr], R0[NRr];
     const int64_t i0 = 2 * Ia - 1;  // first fine plane whose r is needed
     load_u(i0 - 1, Um);
     load_u(i0, U0);
     load_u(i0 + 1, Up);
     load_f(i0, F0);
+    if (D == 2) {
+        load_u(i0 + 2, Up2);
+        load_f(i0 + 1, F1);
+    }
 #pragma unroll
     for (int t = 0; t < NRr; t++) R1[t] = R0[t] = (T)0;
     const uint32_t cplane = (uint32_t)(((c1 - 1) * A.c.s1 + c2) * (int64_t)sizeof(T));
@@ -1214,8 +1219,8 @@ __device__ __forceinline__ void resid_fw_march(const RFArgs<T>& A, int tl) {
     const bool k_out = lane >= 2 && lane <= 60 && (lane & 1) == 0 && K < c2;
     const uint32_t ckoff = (uint32_t)(min(max(K, (int64_t)0), c2 - 1) * (int64_t)sizeof(T));
     for (int64_t i = i0; i <= 2 * Ib - 1; i++) {
-        load_u(i + 2, UL);
-        load_f(i + 1, FL);
+        load_u(i + 1 + D, UL);
+        load_f(i + D, FL);
         // r(i): planes outside the shape are never emitted directly (clamped below)
 #pragma unroll
         for (int t = 0; t < NRr; t++) R2[t] = R1[t], R1[t] = R0[t];
@@ -1256,13 +1261,20 @@ __device__ __forceinline__ void resid_fw_march(const RFArgs<T>& A, int tl) {
         for (int t = 0; t < NU; t++) {
             Um[t] = U0[t];
             U0[t] = Up[t];
-            Up[t] = UL[t];
-            F0[t] = FL[t];
+            if (D == 2) {
+                Up[t] = Up2[t];
+                Up2[t] = UL[t];
+                F0[t] = F1[t];
+                F1[t] = FL[t];
+            } else {
+                Up[t] = UL[t];
+                F0[t] = FL[t];
+            }
         }
     }
 }
 
-template <class T, int RJ>
+template <class T, int RJ, int D>
 __global__ void __launch_bounds__(64) k_resid_fw_wave(RFArgs<T> A) {
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
@@ -1276,8 +1288,8 @@ __global__ void __launch_bounds__(64) k_resid_fw_wave(RFArgs<T> A) {
     const bool inner = 2 * J0 - 1 >= A.by0 && 2 * (J0 + RJ) - 1 < A.by1 && 2 * K0 - 1 >= A.bz0 &&
                        2 * K0 + 2 * FWK - 1 < A.bz1 && 2 * Ia - 1 >= A.bx0 && 2 * Ib - 1 < A.bx1 &&
                        J0 + RJ <= A.c.n1 && K0 + FWK <= A.c.n2;
-    if (inner) resid_fw_march<T, RJ, false>(A, tl);
-    else resid_fw_march<T, RJ, true>(A, tl);
+    if (inner) resid_fw_march<T, RJ, D, false>(A, tl);
+    else resid_fw_march<T, RJ, D, true>(A, tl);
 }
 
 }  // namespace
@@ -1506,7 +1518,10 @@ int march_residual_fw(const mgp_array* u, const mgp_array* f, const mgp_array* r
     if (plane_bytes >= (int64_t)BUF_OOB || cplane >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
     A.hsq = (T)(h * h);  // include/residual.h:23
     A.inv_h2 = (T)(1.0 / (h * h));
-    constexpr int RJ = 4;
+    // measured at 1025 x 1025 x 513: 6 rows for both types, two planes in
+    // flight for FP32 (FP64 at 6 rows and depth 2 spills)
+    const int RJ = g_fw_rows == 4 ? 4 : 6;
+    const int depth = g_fw_depth == 1 || g_fw_depth == 2 ? (int)g_fw_depth : (sizeof(T) == 8 ? 1 : 2);
     const int64_t c0 = cf->shape[0];
     A.ytiles = (int)((cf->shape[1] + RJ - 1) / RJ);
     A.ztiles = (int)((cf->shape[2] + FWK - 1) / FWK);
@@ -1516,7 +1531,14 @@ int march_residual_fw(const mgp_array* u, const mgp_array* f, const mgp_array* r
     A.ichunk = (c0 + chunks - 1) / chunks;
     A.ntiles = (int)(tiles * ((c0 + A.ichunk - 1) / A.ichunk));
     A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
-    hipLaunchKernelGGL((k_resid_fw_wave<T, RJ>), dim3(A.ntiles_pad), dim3(64), 0, (hipStream_t)stream, A);
+    const hipStream_t st = (hipStream_t)stream;
+    if (RJ == 6) {
+        if (depth == 2) hipLaunchKernelGGL((k_resid_fw_wave<T, 6, 2>), dim3(A.ntiles_pad), dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_resid_fw_wave<T, 6, 1>), dim3(A.ntiles_pad), dim3(64), 0, st, A);
+    } else {
+        if (depth == 2) hipLaunchKernelGGL((k_resid_fw_wave<T, 4, 2>), dim3(A.ntiles_pad), dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_resid_fw_wave<T, 4, 1>), dim3(A.ntiles_pad), dim3(64), 0, st, A);
+    }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? MGP_OK : (int)e;
 }
@@ -1524,6 +1546,9 @@ template int march_residual_fw<double>(const mgp_array*, const mgp_array*, const
                                        const int64_t[6], double, void*);
 template int march_residual_fw<float>(const mgp_array*, const mgp_array*, const mgp_array*, const mgp_array*,
                                       const int64_t[6], double, void*);
+
+int64_t g_fw_rows = 0;   // coarse rows per residual + full-weighting wave tile: 4 or 6; 0: 6
+int64_t g_fw_depth = 0;  // fine planes of loads in flight per wave: 1 or 2; 0: 1 (FP64), 2 (FP32)
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
 int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch
@@ -1551,7 +1576,8 @@ template int march_sweep<float>(int, const mgp_array*, const mgp_array*, const m
                                 const int64_t[6], double, int64_t, void*);
 
 int64_t g_sweep_pair = 1;   // relax_level runs sweeps in pairs through the two-sweep pass
-int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, FP32 16, 20 or 24; 0: by type
+int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, FP32 16, 20 or 24; 0: 16
+                            // (measured best for both at 1025 x 1025 x 513)
 
 template <class T>
 int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
@@ -1573,7 +1599,7 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     A.hsq = (T)(faces->h * faces->h);
     int rows = (int)g_sweep2_rows;
     if (sizeof(T) == 8 && rows != 12 && rows != 16) rows = 16;
-    if (sizeof(T) == 4 && rows != 16 && rows != 20 && rows != 24) rows = 20;
+    if (sizeof(T) == 4 && rows != 16 && rows != 20 && rows != 24) rows = 16;
     A.ytiles = (int)((b[3] - b[2] + rows - 9) / (rows - 8));
     A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;

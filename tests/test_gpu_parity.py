@@ -705,13 +705,15 @@ RFW_SHAPES = [((65, 33, 129), "NNNNDD"), ((17, 33, 65), "DDDDDD"), ((33, 17, 65)
               ((5, 5, 5), "NNNNNN"), ((129, 9, 17), "DDNNDN")]
 
 
+@pytest.mark.parametrize("tile", [(4, 1), (4, 2), (6, 1), (6, 2)], ids=["r4d1", "r4d2", "r6d1", "r6d2"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("shape,bc", RFW_SHAPES, ids=[str(i) for i in range(len(RFW_SHAPES))])
-def test_fused_residual_restriction_equals_separate(shape, bc, dtype):
+def test_fused_residual_restriction_equals_separate(shape, bc, dtype, tile):
     """mgp_residual_restrict_fw (residual in registers, full weighting of it,
     r outside the box read as stored) == mgp_residual then mgp_restrict_fw,
-    bit for bit over the whole coarse shape, consistent mode."""
-    say("case resid-fw", shape, bc, dtype.__name__)
+    bit for bit over the whole coarse shape, consistent mode, for every tile
+    height and prefetch depth."""
+    say("case resid-fw", shape, bc, dtype.__name__, tile)
     halo = [1 if c == "N" else 0 for c in bc]
     cshape = tuple((n - 1) // 2 + 1 for n in shape)
     rng = np.random.default_rng(zlib.crc32(repr((shape, bc)).encode()))
@@ -730,8 +732,9 @@ def test_fused_residual_restriction_equals_separate(shape, bc, dtype):
     want = c.download()
     r2 = mgp.DeviceArray(shape, halo, dtype=dtype).upload(r0)
     c2 = mgp.DeviceArray(cshape, halo, dtype=dtype).upload(c0)
-    mgp.call("mgp_residual_restrict_fw_" + sfx, mgp.CONSISTENT, u.ref(), f.ref(), r2.ref(), c2.ref(), mgp.box(box), h,
-             None)
+    with tuned(residual_fw__rows=tile[0], residual_fw__depth=tile[1]):
+        mgp.call("mgp_residual_restrict_fw_" + sfx, mgp.CONSISTENT, u.ref(), f.ref(), r2.ref(), c2.ref(), mgp.box(box),
+                 h, None)
     got = c2.download()
     hN, hS, hE, hW, hT, hB = halo
     sl = (slice(hW, hW + cshape[0]), slice(hS, hS + cshape[1]), slice(hB, hB + cshape[2]))
