@@ -101,6 +101,19 @@ int mgp_cast_f64_f32(const mgp_array* in, const mgp_array* out, void* stream);
 int mgp_cast_f32_f64(const mgp_array* in, const mgp_array* out, void* stream);
 /* u += (double)e over the shape: the FP64 update of iterative refinement */
 int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* stream);
+/* The FP64 update and residual of one refinement iteration in one pass:
+ * u_new = u + (double)e over the shape (u_new a second buffer of u's layout;
+ * its halo is not written), then out = (float)(f - L u_new) on the box, the
+ * Neumann ghosts formed from u_new as the boundary refresh would store them
+ * (include/residual.h:18-81 after include/neumann.h:112-130), and with norm
+ * max |r| over the box and r's stored values outside it.  Equals
+ * mgp_add_f32_to_f64, mgp_bc_neumann_f64 on each Neumann face and
+ * mgp_residual_cast_f64_f32, bit for bit, in 32 instead of 40 bytes per point.
+ * The kernel the survey's proposed ABI calls mgp_update_residual_f64_f32; no
+ * counterpart in the reference (FP64 only there). */
+int mgp_update_residual_f64_f32(int mode, const mgp_array* u, const mgp_array* u_new, const mgp_array* e,
+                                const mgp_array* f, const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
+                                const int64_t box[6], double* norm, void* stream);
 
 /* ------------------------------------------------------------ smoothers */
 /* One red (colour 0) or black (colour 1) half-sweep in place over the box:

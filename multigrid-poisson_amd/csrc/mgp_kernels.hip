@@ -860,6 +860,28 @@ int mgp_cast_f32_f64(const mgp_array* in, const mgp_array* out, void* s) {
     return launch_status();
 }
 
+int mgp_update_residual_f64_f32(int m, const mgp_array* u, const mgp_array* u_new, const mgp_array* e,
+                                const mgp_array* f, const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
+                                const int64_t b[6], double* norm, void* s) {
+    if (!valid_array(u) || !valid_array(u_new) || !valid_array(e) || !valid_array(f) || !valid_array(r) ||
+        !valid_array(out) || !faces || !same_shape(u, u_new) || !same_shape(u, e) || !same_shape(u, f) ||
+        !same_shape(u, r) || !same_shape(u, out) || u->data == u_new->data || !valid_box(u, b) ||
+        (m != MGP_MODE_REFERENCE && m != MGP_MODE_CONSISTENT) || !(faces->h > 0.0))
+        return MGP_ERR_ARG;
+    // a box point on a face needs that face's ghost: Neumann with its data
+    const Box bx = box_of(b);
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == u->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == u->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == u->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
+    return march_update_residual(m, u, u_new, e, f, r, out, faces, b, norm, s);
+}
+
 int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* s) {
     if (!valid_array(u) || !valid_array(e) || !same_shape(u, e)) return MGP_ERR_ARG;
     const Box bx = shape_box(u);

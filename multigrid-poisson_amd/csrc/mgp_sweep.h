@@ -28,6 +28,12 @@ int march_residual(int mode, const mgp_array* u, const mgp_array* f, const mgp_a
 template <class T, class O>
 int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mgp_array* r, const mgp_array* out,
                       const int64_t box[6], double h, double* norm, void* stream, bool store = true);
+// u_new = u + (double) e over the shape, then (float)(f - L u_new) into out on
+// the box with Neumann ghosts formed from u_new (max |r| into *norm, r's
+// stored values outside the box included); u_new a different buffer
+int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, const mgp_array* e, const mgp_array* f,
+                          const mgp_array* r, const mgp_array* out, const mgp_faces* faces, const int64_t box[6],
+                          double* norm, void* stream);
 // u += trilinear P(coarse) over the fine shape through wave tiles
 template <class T>
 int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stream);

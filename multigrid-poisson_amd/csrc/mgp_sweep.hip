@@ -981,6 +981,205 @@ __global__ void __launch_bounds__(64) k_residual_wave(ResidArgs<T, O> A) {
     }
 }
 
+// FP64 update and residual of iterative refinement in one pass
+// (mgp_update_residual_f64_f32): u_new = u + (double) e written to a second
+// buffer, then r = f - L u_new on the box written as FP32, with max |r|.  The
+// wave march of k_residual_wave, each loaded position updated in registers
+// first; the Neumann ghosts the boundary refresh would store before the
+// residual (include/residual.h:26-34 with include/neumann.h:112-130) are formed
+// from u_new in registers: y ghosts from the rows two apart, z ghosts from the
+// lanes two apart, x-ghost planes from the plane two apart.  The shape points
+// outside the box get u + e from extra blocks after the tiles; u_new's halo is
+// not written.
+struct UpdArgs {
+    View<double> u, un, f, r;
+    View<float> e, out;
+    const double* g[6];
+    double two_h;
+    int64_t bx0, bx1, by0, by1, bz0, bz1;
+    int64_t ylo, yhi, zlo, zhi, xlo, xhi;  // allocated (halo-inclusive) index ranges
+    double hsq, inv_h2;
+    unsigned long long* norm;
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t xchunk;
+};
+
+// shape points outside the box: un = u + (double) e (the order of
+// copy_complement)
+__device__ void update_complement(const UpdArgs& A, int blk) {
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2;
+    const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
+    const int64_t xo = A.bx0 + n0 - A.bx1, yo = A.by0 + n1 - A.by1, zo = A.bz0 + n2 - A.bz1;
+    const int64_t c1 = xo * n1 * n2, c2 = BX * yo * n2, c3 = BX * BY * zo;
+    const int64_t per = (int64_t)blockDim.x * COPY_PER_THREAD;
+    const int64_t end = min(c1 + c2 + c3, (int64_t)(blk + 1) * per);
+    for (int64_t t = (int64_t)blk * per + threadIdx.x; t < end; t += blockDim.x) {
+        int64_t x, y, z;
+        if (t < c1) {
+            const int64_t a = t / (n1 * n2), r = t % (n1 * n2);
+            x = a < A.bx0 ? a : A.bx1 + (a - A.bx0);
+            y = r / n2;
+            z = r % n2;
+        } else if (t < c1 + c2) {
+            const int64_t u = t - c1, a = u / (yo * n2), r = u % (yo * n2), c = r / n2;
+            x = A.bx0 + a;
+            y = c < A.by0 ? c : A.by1 + (c - A.by0);
+            z = r % n2;
+        } else {
+            const int64_t u = t - c1 - c2, a = u / (BY * zo), r = u % (BY * zo), c = r % zo;
+            x = A.bx0 + a;
+            y = A.by0 + r / zo;
+            z = c < A.bz0 ? c : A.bz1 + (c - A.bz0);
+        }
+        A.un.at(x, y, z) = A.u.at(x, y, z) + (double)A.e.at(x, y, z);
+    }
+}
+
+template <int MODE, int R>
+__global__ void __launch_bounds__(64) k_update_residual_wave(UpdArgs A) {
+    typedef double T;
+    constexpr int NR = R + 2;
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        update_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * R;
+    const int64_t z0 = A.bz0 + (int64_t)tz * RZ;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2, s0 = A.u.s0, s1 = A.u.s1;
+    const double two_h = A.two_h;
+    const uint32_t plane_bytes = (uint32_t)(((A.yhi - A.ylo) * s1 + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(T));
+    const uint32_t eplane_bytes = plane_bytes / 2;
+    const int lane = threadIdx.x;
+    const int64_t z = z0 - 1 + lane;
+    const bool lane_out = lane >= 1 && lane <= RZ && z < A.bz1;
+    const bool z_shape = z >= 0 && z < n2;
+    const int64_t zc = min(max(z, A.zlo), A.zhi);
+    const int64_t zs = min(max(z, (int64_t)0), n2 - 1);  // clamped into the shape (face index)
+    const uint32_t zoff = (uint32_t)((zc - A.zlo) * (int64_t)sizeof(T));
+    uint32_t roff[NR];
+    bool yout[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int64_t y = y0 - 1 + t;
+        roff[t] = (uint32_t)((min(max(y, A.ylo), A.yhi) - A.ylo) * s1 * (int64_t)sizeof(T));
+        yout[t] = t >= 1 && t <= R && y < A.by1;
+    }
+    const double* gW = A.g[MGP_WEST];
+    const double* gE = A.g[MGP_EAST];
+    const double* gS = A.g[MGP_SOUTH];
+    const double* gN = A.g[MGP_NORTH];
+    const double* gB = A.g[MGP_BOTTOM];
+    const double* gT = A.g[MGP_TOP];
+    const bool y_ghosts = (y0 - 1 < 0 && gS) || (y0 + R >= n1 && gN);
+    const bool z_ghosts = (z0 - 1 < 0 && gB) || (z0 + RZ >= n2 && gT);
+    const int64_t corner = A.ylo * s1 + A.zlo;
+    // u_new of plane i (clamped into the allocated range) with its in-plane
+    // Neumann ghosts
+    auto load_un = [&](int64_t i, T (&U)[NR]) {
+        const int64_t ic = min(max(i, A.xlo), A.xhi);
+        const __amdgpu_buffer_rsrc_t ru = plane_rsrc(A.u.p + ic * s0 + corner, plane_bytes);
+        const __amdgpu_buffer_rsrc_t re = plane_rsrc(A.e.p + ic * s0 + corner, eplane_bytes);
+#pragma unroll
+        for (int t = 0; t < NR; t++) U[t] = bload<T>(ru, zoff, roff[t]) + (double)bload<float>(re, zoff / 2, roff[t] / 2);
+    };
+    auto ghosts = [&](int64_t i, T (&U)[NR]) {
+        if (i < 0 || i >= n0) return;  // x-ghost planes are formed whole below
+        if (y_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 1 + t;
+                if (t + 2 < NR && y == -1 && gS) {
+                    const T v = (T)-1.0 * two_h * gS[i * n2 + zs] + U[t + 2];  // face (nx, 1, nz)
+                    U[t] = z_shape ? v : U[t];
+                } else if (t >= 2 && y == n1 && gN) {
+                    const T v = (T)1.0 * two_h * gN[i * n2 + zs] + U[t - 2];
+                    U[t] = z_shape ? v : U[t];
+                }
+            }
+        }
+        if (z_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 1 + t;
+                if (y < 0 || y >= n1) continue;
+                if (gB) {
+                    const T v = (T)-1.0 * two_h * gB[i * n1 + y] + lane_next(lane_next(U[t]));  // face (nx, ny, 1)
+                    U[t] = z == -1 ? v : U[t];
+                }
+                if (gT) {
+                    const T v = (T)1.0 * two_h * gT[i * n1 + y] + lane_prev(lane_prev(U[t]));
+                    U[t] = z == n2 ? v : U[t];
+                }
+            }
+        }
+    };
+    auto load_f = [&](int64_t i, T (&F)[NR]) {
+        const __amdgpu_buffer_rsrc_t rs = plane_rsrc(A.f.p + min(max(i, A.xlo), A.xhi) * s0 + corner, plane_bytes);
+#pragma unroll
+        for (int t = 1; t <= R; t++) F[t] = bload<T>(rs, zoff, roff[t]);
+    };
+    T Um1[NR], U0[NR], Up1[NR], UL[NR], F0[NR], FL[NR];
+    load_un(xa - 1, Um1);
+    load_un(xa, U0);
+    load_un(xa + 1, Up1);
+    load_f(xa, F0);
+    ghosts(xa - 1, Um1);
+    ghosts(xa, U0);
+    double vmax = 0.0;
+    for (int64_t i = xa; i < xb; i++) {
+        load_un(i + 2, UL);
+        load_f(i + 1, FL);
+        ghosts(i + 1, Up1);
+        // x-ghost planes from the plane two apart (face layout (1, ny, nz))
+        if (i == 0 && gW) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t yc = min(max(y0 - 1 + t, (int64_t)0), n1 - 1);
+                Um1[t] = (T)-1.0 * two_h * gW[yc * n2 + zs] + Up1[t];
+            }
+        }
+        if (i == n0 - 1 && gE) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t yc = min(max(y0 - 1 + t, (int64_t)0), n1 - 1);
+                Up1[t] = (T)1.0 * two_h * gE[yc * n2 + zs] + Um1[t];
+            }
+        }
+        const int64_t ic = min(max(i, A.xlo), A.xhi);
+        const __amdgpu_buffer_rsrc_t rn = plane_rsrc(A.un.p + ic * s0 + corner, plane_bytes);
+        const __amdgpu_buffer_rsrc_t ro = plane_rsrc(A.out.p + ic * s0 + corner, eplane_bytes);
+#pragma unroll
+        for (int t = 1; t <= R; t++) {
+            const bool out = lane_out && yout[t];
+            bstore(U0[t], rn, out ? zoff : BUF_OOB, roff[t]);
+            const T rv = resid_point<T, MODE>(U0[t], Um1[t], Up1[t], U0[t - 1], U0[t + 1], lane_prev(U0[t]),
+                                              lane_next(U0[t]), F0[t], A.hsq, A.inv_h2);
+            bstore((float)rv, ro, out ? zoff / 2 : BUF_OOB, roff[t] / 2);
+            vmax = out ? fmax(vmax, fabs(rv)) : vmax;
+        }
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            Um1[t] = U0[t];
+            U0[t] = Up1[t];
+            Up1[t] = UL[t];
+            F0[t] = FL[t];
+        }
+    }
+    if (A.norm) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+        if (lane == 0 && vmax > 0.0) atomicMax(A.norm, (unsigned long long)__double_as_longlong(vmax));
+    }
+}
+
 // max |r| over a region (the part of the shape outside the box), into *norm
 template <class T>
 __global__ void k_norm_region(View<T> r, int64_t x0, int64_t x1, int64_t y0, int64_t y1, int64_t z0, int64_t z1,
@@ -1439,6 +1638,79 @@ int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mg
     }
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? MGP_OK : (int)e;
+}
+
+int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, const mgp_array* e, const mgp_array* f,
+                          const mgp_array* r, const mgp_array* out, const mgp_faces* faces, const int64_t b[6],
+                          double* norm, void* stream) {
+    for (int d = 0; d < 3; d++)
+        if (u->stride[d] != un->stride[d] || u->stride[d] != f->stride[d] || u->stride[d] != r->stride[d] ||
+            u->stride[d] != e->stride[d] || u->stride[d] != out->stride[d] || u->shape[d] != e->shape[d] ||
+            u->shape[d] != out->shape[d] || u->shape[d] != un->shape[d])
+            return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (u->halo[q] != un->halo[q] || u->halo[q] != f->halo[q] || u->halo[q] != r->halo[q] ||
+            u->halo[q] != e->halo[q] || u->halo[q] != out->halo[q])
+            return MGP_ERR_ARG;
+    UpdArgs A;
+    A.u = view_of<double>(*u);
+    A.un = view_of<double>(*un);
+    A.f = view_of<double>(*f);
+    A.r = view_of<double>(*r);
+    A.e = view_of<float>(*e);
+    A.out = view_of<float>(*out);
+    for (int q = 0; q < 6; q++)
+        A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const double*>(faces->g[q]) : nullptr;
+    A.two_h = 2.0 * faces->h;  // include/neumann.h:108
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.xlo = -u->halo[MGP_WEST], A.xhi = u->shape[0] - 1 + u->halo[MGP_EAST];
+    A.ylo = -u->halo[MGP_SOUTH], A.yhi = u->shape[1] - 1 + u->halo[MGP_NORTH];
+    A.zlo = -u->halo[MGP_BOTTOM], A.zhi = u->shape[2] - 1 + u->halo[MGP_TOP];
+    const int64_t plane_bytes = ((A.yhi - A.ylo) * u->stride[1] + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(double);
+    if (plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    A.hsq = faces->h * faces->h;  // include/residual.h:23
+    A.inv_h2 = 1.0 / (faces->h * faces->h);
+    A.norm = (unsigned long long*)norm;
+    constexpr int R = 16;
+    const int64_t nx = b[1] - b[0], ny = b[3] - b[2], nz = b[5] - b[4];
+    const hipStream_t st = (hipStream_t)stream;
+    A.ytiles = A.ztiles = A.ntiles = A.ntiles_pad = 0;
+    A.xchunk = 1;
+    if (nx > 0 && ny > 0 && nz > 0) {
+        A.ytiles = (int)((ny + R - 1) / R);
+        A.ztiles = (int)((nz + RZ - 1) / RZ);
+        const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+        int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        A.xchunk = (nx + chunks - 1) / chunks;
+        A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
+        A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    }
+    const int copy = complement_blocks(A.u, b, 64);
+    if (A.ntiles_pad + copy > 0) {
+        const dim3 g(A.ntiles_pad + copy), blk(64);
+        if (mode == MGP_MODE_REFERENCE)
+            hipLaunchKernelGGL((k_update_residual_wave<MGP_MODE_REFERENCE, R>), g, blk, 0, st, A);
+        else
+            hipLaunchKernelGGL((k_update_residual_wave<MGP_MODE_CONSISTENT, R>), g, blk, 0, st, A);
+    }
+    if (norm) {
+        // the shape points outside the box, whose r is read as stored
+        const int64_t n0 = u->shape[0], n1 = u->shape[1], n2 = u->shape[2];
+        struct Reg { int64_t x0, x1, y0, y1, z0, z1; };
+        const Reg regs[6] = {{0, b[0], 0, n1, 0, n2},       {b[1], n0, 0, n1, 0, n2},
+                             {b[0], b[1], 0, b[2], 0, n2},  {b[0], b[1], b[3], n1, 0, n2},
+                             {b[0], b[1], b[2], b[3], 0, b[4]}, {b[0], b[1], b[2], b[3], b[5], n2}};
+        for (const Reg& g : regs) {
+            const int64_t cnt = (g.x1 - g.x0) * (g.y1 - g.y0) * (g.z1 - g.z0);
+            if (g.x1 <= g.x0 || g.y1 <= g.y0 || g.z1 <= g.z0) continue;
+            const int blocks = (int)std::min<int64_t>(1024, (cnt + 255) / 256);
+            hipLaunchKernelGGL(k_norm_region<double>, dim3(blocks), dim3(256), 0, st, A.r, g.x0, g.x1, g.y0, g.y1,
+                               g.z0, g.z1, (unsigned long long*)norm);
+        }
+    }
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MGP_OK : (int)err;
 }
 
 template <class T>

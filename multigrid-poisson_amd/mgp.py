@@ -87,6 +87,8 @@ _SIGS = {
     "mgp_zero_scalars": [vp, i64, vp],
     "mgp_cast_f64_f32": [P(Array), P(Array), vp], "mgp_cast_f32_f64": [P(Array), P(Array), vp],
     "mgp_add_f32_to_f64": [P(Array), P(Array), vp],
+    "mgp_update_residual_f64_f32": [i32, P(Array), P(Array), P(Array), P(Array), P(Array), P(Array), P(Faces), BOX,
+                                    vp, vp],
     "mgp_graph_begin": [vp], "mgp_graph_end": [vp, P(vp)], "mgp_graph_launch": [vp, vp], "mgp_graph_destroy": [vp],
     "mgp_event_create": [P(vp)], "mgp_event_destroy": [vp], "mgp_event_record": [vp, vp], "mgp_event_sync": [vp],
     "mgp_event_elapsed_ms": [vp, vp, P(f)],

@@ -142,13 +142,16 @@ public:
         d32.u->init_zero();
         inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
         const mgp_array e32 = d32.u->desc();
-        mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
-        // optional FP64 smoothing of the updated solution: settles u at a
-        // fixed point of the FP64 smoother, whose residual sits below the
-        // rounding floor of an unsmoothed FP64 u at fine h
-        if (settings.refine_polish > 0) outer->smooth_finest(settings.refine_polish, omega);
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
-        residual_norm_to_f32(outer->get_domain(0, 0), f32, dev_norm);
+        // update and residual in one pass where nothing runs between them
+        if (settings.refine_polish > 0 || !update_residual_norm_to_f32(d64, e32, f32, dev_norm)) {
+            mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
+            // optional FP64 smoothing of the updated solution: settles u at a
+            // fixed point of the FP64 smoother, whose residual sits below the
+            // rounding floor of an unsmoothed FP64 u at fine h
+            if (settings.refine_polish > 0) outer->smooth_finest(settings.refine_polish, omega);
+            residual_norm_to_f32(outer->get_domain(0, 0), f32, dev_norm);
+        }
         mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
         f32_ready = true;
     }

@@ -72,5 +72,25 @@ inline void residual_norm_to_f32(Domain<double>& domain, const mgp_array& out, d
                                           mgpx::stream()),
                 "residual");
 }
+// addition: the refinement update u += (double) e and the residual of the
+// updated u, written as FP32 into `out`, in one pass
+// (mgp_update_residual_f64_f32); u + e goes to the second u buffer, which
+// becomes u.  Returns false, with nothing done, where the domain has no second
+// buffer.
+inline bool update_residual_norm_to_f32(Domain<double>& domain, const mgp_array& e, const mgp_array& out,
+                                        double* norm_dev) {
+    if (!domain.u_alt) return false;
+    int64_t b[6];
+    domain.box(b);
+    const mgp_array u = domain.u->desc(), f = domain.f->desc(), r = domain.r->desc();
+    mgp_array un = u;
+    un.data = domain.u_alt;
+    const mgp_faces F = domain.faces();
+    mgpx::check(mgp_update_residual_f64_f32(domain.settings.mode, &u, &un, &e, &f, &r, &out, &F, b, norm_dev,
+                                            mgpx::stream()),
+                "update + residual");
+    domain.swap_u_alt();
+    return true;
+}
 }  // namespace Poisson
 #endif

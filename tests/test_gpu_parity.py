@@ -745,6 +745,63 @@ def test_fused_residual_restriction_equals_separate(shape, bc, dtype, tile):
                                                   None) == mgp.ERR_ARG
 
 
+@pytest.mark.parametrize("mode", [mgp.REFERENCE, mgp.CONSISTENT])
+@pytest.mark.parametrize("shape,bc", RESID_SHAPES, ids=[str(i) for i in range(len(RESID_SHAPES))])
+def test_fused_update_residual_equals_separate(shape, bc, mode):
+    """mgp_update_residual_f64_f32 (u + e into a second buffer, residual of it
+    as FP32 with Neumann ghosts formed in registers) == mgp_add_f32_to_f64,
+    mgp_bc_neumann_f64 on each Neumann face (refresh order E W N S T B) and
+    mgp_residual_cast_f64_f32: u_new over the shape, out everywhere and the
+    norm, bit for bit; u, e and u_new's halo untouched."""
+    say("case update-residual", shape, bc, mode)
+    halo = [1 if c == "N" else 0 for c in bc]
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc, mode, "upd")).encode()))
+    ps = mgp.padded_shape(list(shape), halo)
+    u0, e0, f0, r0, o0, a0 = (rng.standard_normal(ps) for _ in range(6))
+    faces, F = {}, mgp.Faces()
+    for q in range(6):
+        n = [shape[0], shape[1], shape[2]]
+        n[{0: 1, 1: 1, 2: 0, 3: 0, 4: 2, 5: 2}[q]] = 1
+        faces[q] = mgp.DeviceBuffer(rng.standard_normal(int(np.prod(n))))
+        F.type[q] = mgp.NEUMANN if bc[q] == "N" else mgp.DIRICHLET
+        F.g[q] = faces[q].ptr if bc[q] == "N" else None
+    h = 1.0 / (shape[0] - 1)
+    F.h = h
+    box = [int(bc[3] == "D"), shape[0] - int(bc[2] == "D"), int(bc[1] == "D"), shape[1] - int(bc[0] == "D"),
+           int(bc[5] == "D"), shape[2] - int(bc[4] == "D")]
+    f = mgp.DeviceArray(shape, halo).upload(f0)
+    r = mgp.DeviceArray(shape, halo).upload(r0)
+    # separate kernels
+    u = mgp.DeviceArray(shape, halo).upload(u0)
+    e = mgp.DeviceArray(shape, halo, dtype=np.float32).upload(e0)
+    out = mgp.DeviceArray(shape, halo, dtype=np.float32).upload(o0)
+    nrm = mgp.DeviceBuffer(np.zeros(1))
+    mgp.call("mgp_add_f32_to_f64", u.ref(), e.ref(), None)
+    for q in (O.E, O.W, O.N, O.S, O.T, O.B):
+        if bc[q] == "N":
+            mgp.call("mgp_bc_neumann_f64", u.ref(), q, faces[q].ptr, h, None)
+    mgp.call("mgp_residual_cast_f64_f32", mode, u.ref(), f.ref(), r.ref(), out.ref(), mgp.box(box), h, nrm.ptr, None)
+    want_u, want_out, want_n = u.download(), out.download(), nrm.download()[0]
+    # one pass
+    u2 = mgp.DeviceArray(shape, halo).upload(u0)
+    un = mgp.DeviceArray(shape, halo).upload(a0)
+    out2 = mgp.DeviceArray(shape, halo, dtype=np.float32).upload(o0)
+    nrm2 = mgp.DeviceBuffer(np.zeros(1))
+    mgp.call("mgp_update_residual_f64_f32", mode, u2.ref(), un.ref(), e.ref(), f.ref(), r.ref(), out2.ref(),
+             ctypes.byref(F), mgp.box(box), nrm2.ptr, None)
+    got_u = un.download()
+    inner = shape_slices(shape, halo)
+    np.testing.assert_array_equal(got_u[inner], want_u[inner])
+    outside = np.ones(ps, bool)
+    outside[inner] = False
+    np.testing.assert_array_equal(got_u[outside], a0[outside])  # u_new's halo not written
+    np.testing.assert_array_equal(out2.download(), want_out)
+    assert nrm2.download()[0] == want_n and want_n > 0
+    np.testing.assert_array_equal(u2.download(), u0)  # u itself not written
+    assert mgp.lib().mgp_update_residual_f64_f32(mode, u2.ref(), u2.ref(), e.ref(), f.ref(), r.ref(), out2.ref(),
+                                                 ctypes.byref(F), mgp.box(box), nrm2.ptr, None) == mgp.ERR_ARG
+
+
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("shape,bc", RESID_SHAPES[:4], ids=[str(i) for i in range(4)])
 def test_norm_only_residual(shape, bc, dtype):
