@@ -275,6 +275,33 @@ def test_consistent_mode_matches_oracle_bitwise(case, flags):
     np.testing.assert_array_equal(s.array("r"), o.array("r"))
 
 
+def test_headline_shape_one_fcycle_equals_oracle():
+    """At the headline shape itself (1025 x 1025 x 513, 7 levels, F-cycle,
+    red-black GS 2+2 with omega 1, full weighting, Neumann x/y, Dirichlet z,
+    u = x^3 y^2 z, 32 coarsest passes): one F-cycle on the GPU (two-sweep
+    passes, fused descent, wave prolongation, LDS coarsest level, x-chunked
+    tiles) equals one F-cycle of the oracle on the host cores, bit for bit: the
+    relative residual and u over the whole array."""
+    say("case headline shape 1025x1025x513, one F-cycle")
+    dims, levels, h = (1025, 1025, 513), 7, 1.0 / 1024
+    s = mgp.Solver(dims, levels, h, (0, 0, 0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL, mgp.FULL_WEIGHTING, 2,
+                   flags=mgp.QUIET, maxiter=1, miniter=1, tolerance=0.0, coarse_sweeps=32)
+    s.to_device()
+    _it, rel, _t = s.solve("Fcycle", 2, 1.0)
+    s.to_host()
+    u = s.array("u").copy()
+    s.close()
+    say("gpu cycle done, relative residual", rel)
+    o = O.Solver(dims, levels, h, (0, 0, 0), "NNNNDD", O.CONSISTENT, O.GAUSS_SEIDEL, O.FULL_WEIGHTING, 2,
+                 threads=len(__import__("os").sched_getaffinity(0)), coarse_sweeps=32)
+    o.to_device()
+    _oit, orel, _ = o.solve(O.FCYCLE, 2, 1.0, maxiter=1, miniter=1, tol=0.0)
+    say("oracle cycle done, relative residual", orel)
+    assert rel == orel
+    assert np.array_equal(u, o.array("u"))
+    o.close()
+
+
 def test_consistent_fcycle_reaches_1e10_at_129():
     """A mid-size consistent F-cycle (129x129x65, Neumann x/y, Dirichlet z,
     u = x^3 y^2 z) converges below 1e-10 and stays bit-identical to the oracle."""
