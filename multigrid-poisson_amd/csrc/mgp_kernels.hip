@@ -242,32 +242,45 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
             pcol[m] = colour_of<MODE>(lin0, ui.s0, ui.s1, i, j, k);
         }
     }
-    __syncthreads();
     const int s1l = q2, s0l = q1 * q2;
+    // Neumann ghosts: sign*2h*g + mirror (include/neumann.h:112-130).  Each
+    // thread owns at most one ghost cell per face (faces of a level this
+    // kernel takes have at most blockDim cells): its LDS index, the mirror's
+    // offset and the constant sign*2h*g are formed once, not per half-sweep.
+    int gc[6], gd[6];
+    T gv[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        gc[q] = -1;
+        gd[q] = 0;
+        gv[q] = (T)0;
+        if (!F.neumann[q]) continue;
+        const int a0 = (q == MGP_EAST || q == MGP_WEST) ? 1 : n0;
+        const int a1 = (q == MGP_NORTH || q == MGP_SOUTH) ? 1 : n1;
+        const int a2 = (q == MGP_TOP || q == MGP_BOTTOM) ? 1 : n2;
+        const int t = tid;
+        if (t >= a0 * a1 * a2) continue;
+        const int i = t / (a1 * a2), j = (t / a2) % a1, k = t % a2;
+        int gi = i, gj = j, gk = k, d = 0;
+        T sign = (T)1.0;
+        switch (q) {
+        case MGP_EAST: gi = n0; d = -2 * s0l; break;
+        case MGP_WEST: gi = -1; d = 2 * s0l; sign = (T)-1.0; break;
+        case MGP_NORTH: gj = n1; d = -2 * s1l; break;
+        case MGP_SOUTH: gj = -1; d = 2 * s1l; sign = (T)-1.0; break;
+        case MGP_TOP: gk = n2; d = -2; break;
+        default: gk = -1; d = 2; sign = (T)-1.0; break;
+        }
+        gc[q] = L(gi, gj, gk);
+        gd[q] = d;
+        gv[q] = sign * F.two_h * F.g[q][t];
+    }
+    __syncthreads();
     for (int64_t s = 0; s < nsweeps; s++) {
         for (int colour = 0; colour < 2; colour++) {
-            // Neumann ghosts: sign*2h*g + mirror (include/neumann.h:112-130)
-            for (int q = 0; q < 6; q++) {
-                if (!F.neumann[q]) continue;
-                const int a0 = (q == MGP_EAST || q == MGP_WEST) ? 1 : n0;
-                const int a1 = (q == MGP_NORTH || q == MGP_SOUTH) ? 1 : n1;
-                const int a2 = (q == MGP_TOP || q == MGP_BOTTOM) ? 1 : n2;
-                for (int t = tid; t < a0 * a1 * a2; t += nt) {
-                    const int i = t / (a1 * a2), j = (t / a2) % a1, k = t % a2;
-                    int gi = i, gj = j, gk = k, d = 0;
-                    T sign = (T)1.0;
-                    switch (q) {
-                    case MGP_EAST: gi = n0; d = -2 * s0l; break;
-                    case MGP_WEST: gi = -1; d = 2 * s0l; sign = (T)-1.0; break;
-                    case MGP_NORTH: gj = n1; d = -2 * s1l; break;
-                    case MGP_SOUTH: gj = -1; d = 2 * s1l; sign = (T)-1.0; break;
-                    case MGP_TOP: gk = n2; d = -2; break;
-                    default: gk = -1; d = 2; sign = (T)-1.0; break;
-                    }
-                    const int c = L(gi, gj, gk);
-                    su[c] = sign * F.two_h * F.g[q][t] + su[c + d];
-                }
-            }
+#pragma unroll
+            for (int q = 0; q < 6; q++)
+                if (gc[q] >= 0) su[gc[q]] = gv[q] + su[gc[q] + gd[q]];
             __syncthreads();
 #pragma unroll
             for (int m = 0; m < MAXP; m++) {
@@ -643,6 +656,8 @@ static int rbgs_sweeps(int mode, const mgp_array* ui, const mgp_array* uo, const
     const size_t lds = lds_sweeps_bytes(ui, sizeof(T));
     if (!lds) return MGP_ERR_ARG;  // too large for one workgroup
     if ((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) > 8 * 1024) return MGP_ERR_ARG;  // 8 points per thread
+    const int64_t n0 = ui->shape[0], n1 = ui->shape[1], n2 = ui->shape[2];
+    if (n1 * n2 > 1024 || n0 * n2 > 1024 || n0 * n1 > 1024) return MGP_ERR_ARG;  // one ghost per thread and face
     if (box_empty(bx) || nsweeps == 0) {
         if (ui->data != uo->data && !box_empty(bx)) return MGP_ERR_ARG;
         return MGP_OK;

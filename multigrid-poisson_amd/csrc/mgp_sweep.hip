@@ -1876,7 +1876,12 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
     const int64_t nx = b[1] - b[0];
-    int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+    // each chunk spends 6 warm-up steps: aim at 16 chunks per tile, between
+    // 4096 waves and the sweep target (measured at 1025 x 1025 x 513 and its
+    // coarser levels: 16384 waves on the finest, ~5000 on 513 x 513 x 257,
+    // 4096 below)
+    const int64_t target = std::min<int64_t>(g_sweep_target_wgs, std::max<int64_t>(4096, 16 * tiles));
+    int64_t chunks = std::max<int64_t>(1, target / std::max<int64_t>(tiles, 1));
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
     A.xchunk = (nx + chunks - 1) / chunks;
     A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
