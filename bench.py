@@ -245,11 +245,11 @@ def interior(a, shape, halo):
 
 
 def mixed_leg(dims, h, args):
-    """Config 3: FP32 cycles inside FP64 refinement vs plain FP64, same grid,
-    levels, F-cycle and coarsest passes.  Problem of config 3 (all-Dirichlet,
-    u = cos(x z^2) sin(y^3)) at tol 1e-10 and at 2e-10, and the headline
+    """FP32 cycles inside FP64 refinement vs plain FP64, same grid, levels,
+    F-cycle and coarsest passes.  The all-Dirichlet problem
+    u = cos(x z^2) sin(y^3) at tol 1e-10 and at 2e-10, and the headline
     problem (Neumann x/y, u = x^3 y^2 z) at 1e-10; max-abs differences of the
-    config-3 solutions from each other and from the analytic u."""
+    all-Dirichlet solutions from each other and from the analytic u."""
     import numpy as np
 
     dof = dims[0] * dims[1] * dims[2]
@@ -369,7 +369,7 @@ def modes_rhs(d, modes):
 
 
 def batch_leg(args):
-    """Config 4: batched FP32-cycle / FP64-refinement solves (Jacobi 6/7, 3+3,
+    """Batched FP32-cycle / FP64-refinement solves (256 problems by default; Jacobi 6/7, 3+3,
     V-cycle, FW, trilinear, zero Dirichlet) vs the same problems solved one at
     a time (mixed and plain FP64, hierarchies resident, a timed sample)."""
     import numpy as np
@@ -428,7 +428,7 @@ def serve_requests(rng, count):
 
 
 def serve_leg(args):
-    """Config 5: the serving loop (FP32-cycle / FP64-refinement F-cycles,
+    """The serving loop (FP32-cycle / FP64-refinement F-cycles,
     tol 1e-8) on requests of 2^k+1 points per axis (k in 5..9), face types
     by seed, Fourier right-hand sides.  Capacity from a burst; p50/p99 at
     Poisson arrivals of 0.75 x that capacity; against one-at-a-time plain

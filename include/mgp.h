@@ -109,8 +109,7 @@ int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* stream);
  * max |r| over the box and r's stored values outside it.  Equals
  * mgp_add_f32_to_f64, mgp_bc_neumann_f64 on each Neumann face and
  * mgp_residual_cast_f64_f32, bit for bit, in 32 instead of 40 bytes per point.
- * The kernel the survey's proposed ABI calls mgp_update_residual_f64_f32; no
- * counterpart in the reference (FP64 only there). */
+ * No counterpart in the reference (FP64 only there). */
 int mgp_update_residual_f64_f32(int mode, const mgp_array* u, const mgp_array* u_new, const mgp_array* e,
                                 const mgp_array* f, const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
                                 const int64_t box[6], double* norm, void* stream);
