@@ -93,8 +93,60 @@ __device__ __forceinline__ double wave_max64(double v) {
     return v;
 }
 
+// max |.| of the FP64 residual or of f per problem: each workgroup walks a
+// contiguous run of items (the lists hold each problem's items together) and
+// keeps the running max of the problem at hand, folding it into the
+// problem's norm (one atomic per wave) only when the problem changes, instead
+// of once per item
+template <int OP>
+__device__ __forceinline__ void batch_norms(const BArgs& a) {
+    const int per = (a.nitems + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int i0 = (int)blockIdx.x * per, i1 = min(a.nitems, i0 + per);
+    int cur = -1;
+    double vmax = 0.0;
+    auto flush = [&]() {
+        vmax = wave_max64(vmax);
+        if ((threadIdx.x & 63) == 0 && vmax > 0.0) {
+            unsigned long long* dst = OP == OP_FNORM ? (unsigned long long*)&a.state[cur].fnorm : &a.state[cur].rnorm;
+            atomicMax(dst, (unsigned long long)__double_as_longlong(vmax));
+        }
+        vmax = 0.0;
+    };
+    for (int it = i0; it < i1; it++) {
+        const BItem item = a.items[it];
+        const int p = a.grids[item.grid].problem;
+        if (p != cur) {  // uniform per workgroup
+            if (cur >= 0) flush();
+            cur = p;
+        }
+        if (!a.state[p].active) continue;
+        const BTop& t = a.tops[p];
+        const int64_t k = item.z0 + threadIdx.x;
+        const int64_t kz1 = t.u.n2;
+        if (OP == OP_RESIDUAL64 ? (k < kz1 - 1 && k >= 1) : k < kz1) {
+            for (int64_t j = item.y0 + threadIdx.y; j < item.y1; j += blockDim.y) {
+                for (int64_t i = item.x0; i < item.x1; i++) {
+                    if (OP == OP_RESIDUAL64) {
+                        const double rv = residual_point<double, MGP_MODE_CONSISTENT>(t.u, t.u.ix(i, j, k),
+                                                                                      t.f.at(i, j, k), t.hsq, t.inv_h2);
+                        t.r.at(i, j, k) = rv;
+                        vmax = fmax(vmax, fabs(rv));
+                    } else {
+                        vmax = fmax(vmax, fabs(t.f.at(i, j, k)));
+                    }
+                }
+            }
+        }
+    }
+    if (cur >= 0) flush();
+}
+
 template <int OP>
 __global__ void __launch_bounds__(256) k_batch(BArgs a) {
+    if (OP == OP_RESIDUAL64 || OP == OP_FNORM) {
+        batch_norms<OP>(a);
+        return;
+    }
     for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
         const BItem item = a.items[it];
         const BGrid& g = a.grids[item.grid];
@@ -104,9 +156,8 @@ __global__ void __launch_bounds__(256) k_batch(BArgs a) {
         const View<float>& us = g.u[a.src];
         const int64_t kz1 = us.n2;  // level 0 of the FP64 arrays has the level-0 shape
         // interior-only operations stop one short of the far face
-        constexpr bool interior = OP == OP_JACOBI || OP == OP_GS_HALF || OP == OP_RESIDUAL || OP == OP_RESIDUAL64;
+        constexpr bool interior = OP == OP_JACOBI || OP == OP_GS_HALF || OP == OP_RESIDUAL;
         const int64_t kmax = interior ? kz1 - 1 : kz1;
-        double vmax = 0.0;
         if (k < kmax && (!interior || k >= 1)) {
             for (int64_t j = item.y0 + threadIdx.y; j < item.y1; j += blockDim.y) {
                 for (int64_t i = item.x0; i < item.x1; i++) {
@@ -154,26 +205,10 @@ __global__ void __launch_bounds__(256) k_batch(BArgs a) {
                     case OP_UPDATE:
                         a.tops[p].u.at(i, j, k) += (double)us.at(i, j, k);
                         break;
-                    case OP_RESIDUAL64: {
-                        const BTop& t = a.tops[p];
-                        const double rv = residual_point<double, MGP_MODE_CONSISTENT>(t.u, t.u.ix(i, j, k),
-                                                                                      t.f.at(i, j, k), t.hsq, t.inv_h2);
-                        t.r.at(i, j, k) = rv;
-                        vmax = fmax(vmax, fabs(rv));
-                        break;
-                    }
-                    case OP_FNORM:
-                        vmax = fmax(vmax, fabs(a.tops[p].f.at(i, j, k)));
+                    default:  // the norms: batch_norms
                         break;
                     }
                 }
-            }
-        }
-        if (OP == OP_RESIDUAL64 || OP == OP_FNORM) {
-            vmax = wave_max64(vmax);
-            if ((threadIdx.x & 63) == 0 && vmax > 0.0) {
-                unsigned long long* dst = OP == OP_FNORM ? (unsigned long long*)&a.state[p].fnorm : &a.state[p].rnorm;
-                atomicMax(dst, (unsigned long long)__double_as_longlong(vmax));
             }
         }
     }
