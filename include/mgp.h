@@ -441,7 +441,8 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * 2 fine planes of loads in flight per wave; 0 = 1 for FP64 and 2 for FP32,
  * default),
  * "prolong.kernel" (1: per-point kernel; 2: wave tiles marching along x for
- * mgp_prolong_add_*, default).
+ * mgp_prolong_add_*, default), "batch.blocks" (workgroups per launch of the
+ * batched solver's work lists, default 16384).
  * No counterpart in the
  * reference, whose launch
  * shape was fixed by the OpenMP runtime. */

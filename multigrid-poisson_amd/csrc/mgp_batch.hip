@@ -258,6 +258,10 @@ struct Tile {
     int xd, yd;  // planes per item, rows per item
 };
 
+// workgroups of a batched launch (fewer when the list has fewer items): the
+// items are small, so many workgroups keep enough loads in flight
+int64_t g_batch_blocks = 16384;
+
 static std::mutex g_tile_mu;
 static std::map<std::tuple<int64_t, int64_t, int64_t>, Tile> g_tile_cache;
 
@@ -643,7 +647,9 @@ private:
         list_top_interior = upload(topi);
     }
 
-    static int persistent_blocks(int nitems) { return std::max(1, std::min(nitems, 256 * 8)); }
+    static int persistent_blocks(int nitems) {
+        return (int)std::max<int64_t>(1, std::min<int64_t>(nitems, g_batch_blocks));
+    }
 
     BArgs args(int op, const BItem* items, int nitems, int src, int src2, int colour) const {
         BArgs a;

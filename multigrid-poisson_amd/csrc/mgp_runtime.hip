@@ -131,6 +131,7 @@ int mgp_tuning_set(const char* key, int64_t value) {
         mgp::g_sweep2_rows = value;
     else if (!strcmp(key, "residual_fw.rows") && (value == 0 || value == 4 || value == 6)) mgp::g_fw_rows = value;
     else if (!strcmp(key, "residual_fw.depth") && (value == 0 || value == 1 || value == 2)) mgp::g_fw_depth = value;
+    else if (!strcmp(key, "batch.blocks") && value >= 1 && value <= (1 << 20)) mgp::g_batch_blocks = value;
     else if (!strcmp(key, "residual.kernel") && (value == 1 || value == 2)) mgp::g_resid_kernel = value;
     else if (!strcmp(key, "residual.rows") && (value == 8 || value == 16)) mgp::g_resid_rows = value;
     else if (!strcmp(key, "prolong.kernel") && (value == 1 || value == 2)) mgp::g_prolong_kernel = value;
@@ -150,6 +151,7 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "sweep2.rows")) *value = mgp::g_sweep2_rows;
     else if (!strcmp(key, "residual_fw.rows")) *value = mgp::g_fw_rows;
     else if (!strcmp(key, "residual_fw.depth")) *value = mgp::g_fw_depth;
+    else if (!strcmp(key, "batch.blocks")) *value = mgp::g_batch_blocks;
     else if (!strcmp(key, "residual.kernel")) *value = mgp::g_resid_kernel;
     else if (!strcmp(key, "residual.rows")) *value = mgp::g_resid_rows;
     else if (!strcmp(key, "prolong.kernel")) *value = mgp::g_prolong_kernel;

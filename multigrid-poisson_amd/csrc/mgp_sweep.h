@@ -49,6 +49,7 @@ extern int64_t g_resid_kernel;
 extern int64_t g_resid_rows;
 // default x-chunk length for a box
 int64_t march_xchunk(const int64_t box[6]);
+extern int64_t g_batch_blocks;  // workgroups per batched launch (mgp_batch.hip)
 extern int64_t g_sweep_target_wgs;
 extern int64_t g_sweep_min_chunk;
 extern int64_t g_sweep_waves;
