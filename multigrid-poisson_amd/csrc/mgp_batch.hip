@@ -45,8 +45,8 @@ struct BGrid {
     float one_omega, omega_sixth, hsq, inv_h2;
 };
 
-struct BTop {  // FP64 finest level of a problem
-    View<double> u, f, r;
+struct BTop {  // FP64 finest level of a problem (its residual goes straight to the FP32 f)
+    View<double> u, f;
     double hsq, inv_h2;
 };
 
@@ -70,9 +70,8 @@ enum BOp : int {
     OP_FW,           // coarse f = FW(fine r) over the coarse shape (grid = coarse level)
     OP_ZERO_U,       // u[src] = 0 over the shape
     OP_PROLONG_ADD,  // u[src] += P(coarse u[src2]) over the fine shape (grid = fine level)
-    OP_CAST,         // level-0 f32 = (float) r64 over the shape
     OP_UPDATE,       // u64 += (double) u[src] over the shape (level 0)
-    OP_RESIDUAL64,   // r64 = f64 - L u64 on the interior, max |r64| per problem
+    OP_RESIDUAL64,   // level-0 f32 = (float)(f64 - L u64) on the interior, max |r| per problem
     OP_FNORM,        // max |f64| per problem over the shape
 };
 
@@ -129,7 +128,7 @@ __device__ __forceinline__ void batch_norms(const BArgs& a) {
                     if (OP == OP_RESIDUAL64) {
                         const double rv = residual_point<double, MGP_MODE_CONSISTENT>(t.u, t.u.ix(i, j, k),
                                                                                       t.f.at(i, j, k), t.hsq, t.inv_h2);
-                        t.r.at(i, j, k) = rv;
+                        a.grids[item.grid].f.at(i, j, k) = (float)rv;  // the next iteration's right-hand side
                         vmax = fmax(vmax, fabs(rv));
                     } else {
                         vmax = fmax(vmax, fabs(t.f.at(i, j, k)));
@@ -199,9 +198,6 @@ __global__ void __launch_bounds__(256) k_batch(BArgs a) {
                         us.at(i, j, k) += trilinear<float>(cg.u[a.src2], i, j, k);
                         break;
                     }
-                    case OP_CAST:
-                        g.f.at(i, j, k) = (float)a.tops[p].r.at(i, j, k);
-                        break;
                     case OP_UPDATE:
                         a.tops[p].u.at(i, j, k) += (double)us.at(i, j, k);
                         break;
@@ -434,7 +430,7 @@ private:
         }
         top_u_bytes = bytes;
         const size_t fr64 = bytes;
-        for (int p = 0; p < n; p++) bytes += 2 * align(count_of(p, 0) * sizeof(double));
+        for (int p = 0; p < n; p++) bytes += align(count_of(p, 0) * sizeof(double));
         for (int g = 0; g < ng; g++) {
             off32[g] = bytes;
             const int64_t c = gshape[g][0] * gshape[g][1] * gshape[g][2];
@@ -465,8 +461,7 @@ private:
             BTop t;
             t.u = view64(off64[p], s);
             t.f = view64(o, s);
-            t.r = view64(o + sz, s);
-            o += 2 * sz;
+            o += sz;
             const double h = 1.0 / (double)(shapes[3 * p] - 1);
             t.hsq = h * h;  // include/residual.h:23
             t.inv_h2 = 1.0 / (h * h);
@@ -672,7 +667,6 @@ private:
         case OP_FW: hipLaunchKernelGGL(k_batch<OP_FW>, grid, dim3(64, 4), 0, s, a); break;
         case OP_ZERO_U: hipLaunchKernelGGL(k_batch<OP_ZERO_U>, grid, dim3(64, 4), 0, s, a); break;
         case OP_PROLONG_ADD: hipLaunchKernelGGL(k_batch<OP_PROLONG_ADD>, grid, dim3(64, 4), 0, s, a); break;
-        case OP_CAST: hipLaunchKernelGGL(k_batch<OP_CAST>, grid, dim3(64, 4), 0, s, a); break;
         case OP_UPDATE: hipLaunchKernelGGL(k_batch<OP_UPDATE>, grid, dim3(64, 4), 0, s, a); break;
         case OP_RESIDUAL64: hipLaunchKernelGGL(k_batch<OP_RESIDUAL64>, grid, dim3(64, 4), 0, s, a); break;
         default: hipLaunchKernelGGL(k_batch<OP_FNORM>, grid, dim3(64, 4), 0, s, a); break;
@@ -697,12 +691,12 @@ private:
         }
     }
 
-    // one refinement iteration: f32 = r64, e = 0, one FP32 V-cycle, u64 += e,
-    // r64 and its norm, the stop rule
+    // one refinement iteration: e = 0, one FP32 V-cycle on f32 (the previous
+    // residual, already FP32), u64 += e, the residual into f32 and its norm,
+    // the stop rule
     void enqueue_iteration() {
         const int64_t nu = cfg.nsmooth;
         const int64_t cs = cfg.coarse_sweeps > 0 ? cfg.coarse_sweeps : nu;
-        launch(OP_CAST, list_top, 0, 0, 0);
         // level-0 e lives in the current buffer of level 0 (group A, or B for
         // one-level problems)
         launch(OP_ZERO_U, shape_a[0], cur[0][0], 0, 0);
