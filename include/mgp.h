@@ -114,6 +114,42 @@ int mgp_update_residual_f64_f32(int mode, const mgp_array* u, const mgp_array* u
                                 const mgp_array* f, const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
                                 const int64_t box[6], double* norm, void* stream);
 
+/* ------------------------------------------- FP16 storage of coarse levels
+ * (no counterpart in the reference: its T is one type throughout).  Arrays
+ * hold IEEE binary16; values are loaded as float, combined with the
+ * consistent-mode arithmetic of the *_f32 kernels (the reference's operators:
+ * gaussseidel.h:28-94, residual.h:18-81, fullweighting.h:37-122,
+ * trilinearinterpolation.h:29-134) and rounded to nearest even when stored.
+ * All faces Dirichlet: boxes lie strictly inside the shape (MGP_ERR_ARG
+ * otherwise), colour = parity of i+j+k.
+ * Scaling: the restriction into an FP16 level writes sigma * FW(r) with
+ * sigma = 2^-e, where *norm_dev = max |r| = m 2^e, m in [0.5,1) (sigma = 1
+ * for a zero norm); the prolongation from that level adds P(u) * 2^e with
+ * the same norm.  Powers of two scale exactly. */
+int mgp_fill_f16(const mgp_array* a, double value, void* stream);
+/* one red (0) or black (1) half-sweep in place */
+int mgp_rbgs_half_f16(const mgp_array* u, const mgp_array* f, const int64_t box[6], double omega, double h,
+                      int colour, void* stream);
+/* nsweeps red-then-black sweeps in place in one launch (u and f in one
+ * workgroup's LDS), equal to 2 nsweeps mgp_rbgs_half_f16 calls; levels of at
+ * most 8192 points (MGP_ERR_ARG otherwise) */
+int mgp_rbgs_sweeps_f16(const mgp_array* u, const mgp_array* f, const int64_t box[6], double omega, double h,
+                        int64_t nsweeps, void* stream);
+/* r = f - L u on the box; with norm_dev, max(*norm_dev, max |r| of the
+ * stored r over its shape) */
+int mgp_residual_f16(const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t box[6], double h,
+                     double* norm_dev, void* stream);
+/* coarse = sigma * FW(fine): FP16 to FP16, and FP32 to FP16 (from the last
+ * FP32 level) */
+int mgp_restrict_fw_scaled_f16(const mgp_array* fine, const mgp_array* coarse, const double* norm_dev, void* stream);
+int mgp_restrict_fw_scaled_f32_f16(const mgp_array* fine, const mgp_array* coarse, const double* norm_dev,
+                                   void* stream);
+/* fine += P(coarse) / sigma over the fine shape: FP16 to FP16, and FP16 to
+ * FP32 (into the last FP32 level) */
+int mgp_prolong_add_scaled_f16(const mgp_array* coarse, const mgp_array* fine, const double* norm_dev, void* stream);
+int mgp_prolong_add_scaled_f16_f32(const mgp_array* coarse, const mgp_array* fine, const double* norm_dev,
+                                   void* stream);
+
 /* ------------------------------------------------------------ smoothers */
 /* One red (colour 0) or black (colour 1) half-sweep in place over the box:
  * u = (1-w)u + w/6 (sum_nb u - h^2 f) (include/gaussseidel.h:28-94).  Halos
@@ -248,6 +284,9 @@ typedef struct mgp_solver_config {
     int64_t coarse_sweeps; /* Settings::coarse_sweeps (consistent mode; 0 = nsmooth) */
     int64_t refine_polish; /* MGP_MIXED: FP64 smoothing passes on the finest level after each
                               update (Settings::refine_polish; 0 = none) */
+    int64_t half_from;     /* MGP_MIXED: levels >= half_from of the FP32 cycle stored in FP16
+                              (Settings::half_from; 0 = none; needs 2 <= half_from < levels,
+                              Gauss-Seidel, full weighting, all faces Dirichlet) */
 } mgp_solver_config;
 
 /* PoissonSolver(settings, BC) (libpoisson.h:79-87) */

@@ -482,6 +482,177 @@ __global__ void k_bc_restrict_neumann(int restriction, View<T> uf, int loc, View
 
 // ======================================================= host-side wrappers
 
+// ============================================== FP16 storage, FP32 arithmetic
+//
+// The coarse levels of an FP32 cycle may be stored as IEEE binary16 (an
+// addition; the reference computes in one type T throughout).  Values are
+// loaded as float, combined with the consistent-mode point arithmetic of the
+// FP32 kernels, and rounded to binary16 (nearest even) when stored.  All
+// faces Dirichlet: the boxes stay one point inside the shape.
+//
+// Scaling: a level's right-hand side is the restricted residual of the level
+// above times sigma = 2^-e, where max |r| = m 2^e with m in [0.5, 1) (read
+// on the device from the norm the residual kernel wrote), so restricted
+// values stay near 1 whatever the refinement has reduced the residual to;
+// the prolongation multiplies the correction by 2^e again.  Powers of two
+// scale exactly, so the scaled problem is the same linear problem.
+
+// v rounded to float, then to binary16 (nearest even): the empty asm keeps
+// the compiler from folding the operation that produced v into the
+// conversion (v_fma_mixlo_f16 and the like round once, straight to binary16)
+__device__ __forceinline__ _Float16 to_half(float v) {
+    asm volatile("" : "+v"(v));
+    return (_Float16)v;
+}
+
+struct HalfView {
+    _Float16* p;  // element (0,0,0) of the shape
+    int64_t n0, n1, n2, s0, s1;
+    explicit HalfView(const View<_Float16>& v) : p(v.p), n0(v.n0), n1(v.n1), n2(v.n2), s0(v.s0), s1(v.s1) {}
+    __device__ __forceinline__ int64_t ix(int64_t i, int64_t j, int64_t k) const { return i * s0 + j * s1 + k; }
+    __device__ __forceinline__ float ld(int64_t c) const { return (float)p[c]; }
+    __device__ __forceinline__ float at(int64_t i, int64_t j, int64_t k) const { return (float)p[ix(i, j, k)]; }
+    __device__ __forceinline__ void st(int64_t c, float v) const { p[c] = to_half(v); }
+};
+
+__device__ __forceinline__ void add_into(const View<float>& a, int64_t i, int64_t j, int64_t k, float v) {
+    a.at(i, j, k) += v;
+}
+__device__ __forceinline__ void add_into(const HalfView& a, int64_t i, int64_t j, int64_t k, float v) {
+    const int64_t c = a.ix(i, j, k);
+    a.st(c, a.ld(c) + v);
+}
+
+// 2^-e (or 2^e with `inverse`) for the max |r| in *norm; 1 for a zero or
+// non-finite norm
+__device__ __forceinline__ float pow2_scale(const double* norm, bool inverse) {
+    const double n = *norm;
+    if (!(n > 0.0) || !(n < 1e300)) return 1.0f;
+    int e;
+    (void)frexp(n, &e);
+    e = min(120, max(-120, e));
+    return ldexpf(1.0f, inverse ? e : -e);
+}
+
+// consistent-mode residual_point on FP16 storage
+__device__ __forceinline__ float residual_point_h(const HalfView& u, int64_t c, float fc, float inv_h2) {
+    const float uc = u.ld(c);
+    const float s = ((u.ld(c - u.s0) - uc) + (u.ld(c + u.s0) - uc)) + ((u.ld(c - u.s1) - uc) + (u.ld(c + u.s1) - uc)) +
+                    ((u.ld(c - 1) - uc) + (u.ld(c + 1) - uc));
+    return fmaf(-inv_h2, s, fc);
+}
+
+__global__ void k_fill_h(_Float16* p, int64_t n, float v) {
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const _Float16 hv = to_half(v);
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < n; t += stride) p[t] = hv;
+}
+
+// one red (colour 0) or black half-sweep in place, colour = parity of i+j+k
+__global__ void k_gs_half_h(HalfView u, HalfView f, Box bx, int64_t zc, float omega_sixth, float hsq, int colour) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    if ((int)((i + j + k) & 1) != colour) return;
+    const int64_t c = u.ix(i, j, k);
+    u.st(c, relax_point<float, MGP_MODE_CONSISTENT>(u.ld(c), u.ld(c - u.s0), u.ld(c + u.s0), u.ld(c - u.s1),
+                                                    u.ld(c + u.s1), u.ld(c - 1), u.ld(c + 1), f.at(i, j, k), 0.0f,
+                                                    omega_sixth, hsq));
+}
+
+// nsweeps red-black sweeps of a level that fits one workgroup's LDS, in one
+// launch: u and f staged as float, every update rounded to binary16 as the
+// per-colour kernel stores it, so the result equals 2 nsweeps launches of
+// k_gs_half_h
+__global__ void __launch_bounds__(1024) k_sweeps_lds_h(HalfView u, HalfView f, Box bx, int64_t nsweeps,
+                                                        float omega_sixth, float hsq) {
+    constexpr int MAXP = 8;  // box points per thread (<= 8192 points)
+    extern __shared__ unsigned char lds_raw[];
+    float* su = reinterpret_cast<float*>(lds_raw);
+    const int n0 = (int)u.n0, n1 = (int)u.n1, n2 = (int)u.n2;
+    const int N = n0 * n1 * n2;
+    float* sf = su + N;
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < N; t += nt) {
+        const int i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
+        su[t] = u.at(i, j, k);
+        sf[t] = f.at(i, j, k);
+    }
+    const int bn0 = (int)(bx.x1 - bx.x0), bn1 = (int)(bx.y1 - bx.y0), bn2 = (int)(bx.z1 - bx.z0);
+    const int nbox = bn0 * bn1 * bn2;
+    int pc[MAXP], pcol[MAXP];
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        const int t = tid + m * nt;
+        pc[m] = -1;
+        pcol[m] = 2;
+        if (t < nbox) {
+            const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
+            pc[m] = (i * n1 + j) * n2 + k;
+            pcol[m] = (i + j + k) & 1;
+        }
+    }
+    const int s1l = n2, s0l = n1 * n2;
+    __syncthreads();
+    for (int64_t s = 0; s < nsweeps; s++) {
+        for (int colour = 0; colour < 2; colour++) {
+#pragma unroll
+            for (int m = 0; m < MAXP; m++) {
+                if (pcol[m] != colour) continue;
+                const int c = pc[m];
+                const float v = relax_point<float, MGP_MODE_CONSISTENT>(su[c], su[c - s0l], su[c + s0l], su[c - s1l],
+                                                                        su[c + s1l], su[c - 1], su[c + 1], sf[c],
+                                                                        0.0f, omega_sixth, hsq);
+                su[c] = (float)to_half(v);
+            }
+            __syncthreads();
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        if (pc[m] < 0) continue;
+        const int c = pc[m];
+        const int i = c / (n1 * n2), j = (c / n2) % n1, k = c % n2;
+        u.st(u.ix(i, j, k), su[c]);
+    }
+}
+
+// r = f - L u on the box (stored as binary16) and max |r| of the stored
+// values over r's whole shape into *out (columns over the shape, as
+// k_residual_norm)
+__global__ void k_residual_norm_h(HalfView u, HalfView f, HalfView r, Box shape, Box bx, float inv_h2,
+                                  unsigned long long* out) {
+    int64_t j, k;
+    double v = 0.0;
+    if (column_point(shape, j, k)) {
+        const bool col_in = j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1;
+        for (int64_t i = shape.x0; i < shape.x1; i++) {
+            const int64_t c = r.ix(i, j, k);
+            if (col_in && i >= bx.x0 && i < bx.x1) r.st(c, residual_point_h(u, u.ix(i, j, k), f.at(i, j, k), inv_h2));
+            v = fmax(v, fabs((double)r.ld(c)));
+        }
+    }
+    if (out) block_max_into(v, out);
+}
+
+// coarse f = sigma * FW(fine r), fine r in FP32 (the last FP32 level) or FP16
+template <class V>
+__global__ void k_fw_scaled_h(V in, HalfView out, Box bx, int64_t zc, const double* norm) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const float sigma = pow2_scale(norm, false);
+    out.st(out.ix(i, j, k), fw_point<float, V>(in, i, j, k) * sigma);
+}
+
+// fine u += P(coarse u) / sigma over the fine shape, fine u in FP32 or FP16
+template <class V>
+__global__ void k_prolong_add_scaled_h(HalfView coarse, V fine, Box bx, int64_t zc, const double* norm) {
+    int64_t i, j, k;
+    if (!box_point(bx, zc, i, j, k)) return;
+    const float inv = pow2_scale(norm, true);
+    add_into(fine, i, j, k, trilinear<float, HalfView>(coarse, i, j, k) * inv);
+}
+
+
 static inline int launch_status() {
     const hipError_t e = hipGetLastError();
     return e == hipSuccess ? MGP_OK : (int)e;
@@ -843,6 +1014,98 @@ static int residual_restrict_fw(int mode, const mgp_array* u, const mgp_array* f
     return march_residual_fw<T>(u, f, r, cf, b, h, stream);
 }
 
+
+// ---- FP16 storage: validation and launches
+static inline HalfView half_view(const mgp_array* a) { return HalfView(view_of<_Float16>(*a)); }
+
+// box strictly inside the shape (all faces Dirichlet)
+static inline bool interior_box(const mgp_array* a, const int64_t b[6]) {
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < b[2 * d + 1] && (b[2 * d] < 1 || b[2 * d + 1] > a->shape[d] - 1)) return false;
+    return true;
+}
+
+static int fill_f16(const mgp_array* a, double value, void* stream) {
+    if (!a || !a->data) return MGP_ERR_ARG;
+    const int64_t n = a->stride[0] * (a->shape[0] + a->halo[MGP_EAST] + a->halo[MGP_WEST]);
+    const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_fill_h, dim3((unsigned)blocks), dim3(256), 0, (hipStream_t)stream, (_Float16*)a->data, n,
+                       (float)value);
+    return launch_status();
+}
+
+static int rbgs_half_f16(const mgp_array* u, const mgp_array* f, const int64_t b[6], double omega, double h,
+                         int colour, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !same_shape(u, f) || !interior_box(u, b) || (colour & ~1))
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    if (box_empty(bx)) return MGP_OK;
+    const BoxGrid g = box_grid(bx);
+    const RelaxConsts<float> c(omega, h);
+    hipLaunchKernelGGL(k_gs_half_h, g.grid, g.block, 0, (hipStream_t)stream, half_view(u), half_view(f), bx, g.zchunks,
+                       c.omega_sixth, c.hsq, colour);
+    return launch_status();
+}
+
+static int rbgs_sweeps_f16(const mgp_array* u, const mgp_array* f, const int64_t b[6], double omega, double h,
+                           int64_t nsweeps, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !same_shape(u, f) || !interior_box(u, b) || nsweeps < 0)
+        return MGP_ERR_ARG;
+    const int64_t N = u->shape[0] * u->shape[1] * u->shape[2];
+    if (N > 8192) return MGP_ERR_ARG;  // u and f as float in 64 KiB of LDS, 8 box points per thread
+    const Box bx = box_of(b);
+    if (box_empty(bx) || nsweeps == 0) return MGP_OK;
+    const RelaxConsts<float> c(omega, h);
+    hipLaunchKernelGGL(k_sweeps_lds_h, dim3(1), dim3(1024), (size_t)(2 * N * sizeof(float)), (hipStream_t)stream,
+                       half_view(u), half_view(f), bx, nsweeps, c.omega_sixth, c.hsq);
+    return launch_status();
+}
+
+static int residual_f16(const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6], double h,
+                        double* norm, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !valid_array(r) || !same_shape(u, f) || !same_shape(u, r) ||
+        !interior_box(u, b))
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b), sh = shape_box(r);
+    const BoxGrid g = column_grid(sh);
+    const float inv_h2 = (float)(1.0 / (h * h));  // as the FP32 residual forms it
+    hipLaunchKernelGGL(k_residual_norm_h, g.grid, g.block, 0, (hipStream_t)stream, half_view(u), half_view(f),
+                       half_view(r), sh, bx, inv_h2, (unsigned long long*)norm);
+    return launch_status();
+}
+
+static int restrict_fw_scaled(bool from_f32, const mgp_array* in, const mgp_array* out, const double* norm,
+                              void* stream) {
+    if (!valid_array(in) || !valid_array(out) || !coarse_of(in, out) || !norm) return MGP_ERR_ARG;
+    const Box bx = shape_box(out);
+    const BoxGrid g = box_grid(bx);
+    const hipStream_t s = (hipStream_t)stream;
+    if (from_f32)
+        hipLaunchKernelGGL(k_fw_scaled_h<View<float>>, g.grid, g.block, 0, s, view_of<float>(*in), half_view(out), bx,
+                           g.zchunks, norm);
+    else
+        hipLaunchKernelGGL(k_fw_scaled_h<HalfView>, g.grid, g.block, 0, s, half_view(in), half_view(out), bx,
+                           g.zchunks, norm);
+    return launch_status();
+}
+
+static int prolong_add_scaled(bool to_f32, const mgp_array* coarse, const mgp_array* fine, const double* norm,
+                              void* stream) {
+    if (!valid_array(coarse) || !valid_array(fine) || !norm) return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+        if (2 * (coarse->shape[d] - 1) + 1 != fine->shape[d]) return MGP_ERR_ARG;
+    const Box bx = shape_box(fine);
+    const BoxGrid g = box_grid(bx);
+    const hipStream_t s = (hipStream_t)stream;
+    if (to_f32)
+        hipLaunchKernelGGL(k_prolong_add_scaled_h<View<float>>, g.grid, g.block, 0, s, half_view(coarse),
+                           view_of<float>(*fine), bx, g.zchunks, norm);
+    else
+        hipLaunchKernelGGL(k_prolong_add_scaled_h<HalfView>, g.grid, g.block, 0, s, half_view(coarse),
+                           half_view(fine), bx, g.zchunks, norm);
+    return launch_status();
+}
+
 }  // namespace mgp
 
 // ============================================================ C entry points
@@ -1041,6 +1304,32 @@ int mgp_bc_restrict_neumann_f64(int r, const mgp_array* u, int l, const void* gf
 }
 int mgp_bc_restrict_neumann_f32(int r, const mgp_array* u, int l, const void* gf, void* gc, double h, void* s) {
     return bc_restrict_neumann<float>(r, u, l, gf, gc, h, s);
+}
+
+// FP16 storage of coarse levels (consistent arithmetic, Dirichlet faces)
+int mgp_fill_f16(const mgp_array* a, double v, void* s) { return fill_f16(a, v, s); }
+int mgp_rbgs_half_f16(const mgp_array* u, const mgp_array* f, const int64_t b[6], double w, double h, int c, void* s) {
+    return rbgs_half_f16(u, f, b, w, h, c, s);
+}
+int mgp_rbgs_sweeps_f16(const mgp_array* u, const mgp_array* f, const int64_t b[6], double w, double h, int64_t n,
+                        void* s) {
+    return rbgs_sweeps_f16(u, f, b, w, h, n, s);
+}
+int mgp_residual_f16(const mgp_array* u, const mgp_array* f, const mgp_array* r, const int64_t b[6], double h,
+                     double* norm, void* s) {
+    return residual_f16(u, f, r, b, h, norm, s);
+}
+int mgp_restrict_fw_scaled_f16(const mgp_array* i, const mgp_array* o, const double* norm, void* s) {
+    return restrict_fw_scaled(false, i, o, norm, s);
+}
+int mgp_restrict_fw_scaled_f32_f16(const mgp_array* i, const mgp_array* o, const double* norm, void* s) {
+    return restrict_fw_scaled(true, i, o, norm, s);
+}
+int mgp_prolong_add_scaled_f16(const mgp_array* c, const mgp_array* u, const double* norm, void* s) {
+    return prolong_add_scaled(false, c, u, norm, s);
+}
+int mgp_prolong_add_scaled_f16_f32(const mgp_array* c, const mgp_array* u, const double* norm, void* s) {
+    return prolong_add_scaled(true, c, u, norm, s);
 }
 
 }  // extern "C"

@@ -55,12 +55,15 @@ __device__ __forceinline__ T residual_point(const View<T>& u, int64_t c, T fc, T
     }
 }
 
+// The grid transfers below take any view type V with at(i,j,k) and n0..n2
+// (View<T>, or FP16 storage read as float).
+//
 // Full weighting at coarse point (i,j,k): (1,2,1)^3/64 of the fine values
 // with neighbour indices clamped to [0, n-1]; the three z-planes summed in the
 // reference's term order (T tmp = plane; tmp += plane),
 // include/fullweighting.h:59-121
-template <class T>
-__device__ __forceinline__ T fw_point(const View<T>& in, int64_t i, int64_t j, int64_t k) {
+template <class T, class V = View<T>>
+__device__ __forceinline__ T fw_point(const V& in, int64_t i, int64_t j, int64_t k) {
     const int64_t n0 = in.n0, n1 = in.n1, n2 = in.n2;
     const int64_t ii = min(n0 - 1, 2 * i), jj = min(n1 - 1, 2 * j), kk = min(n2 - 1, 2 * k);
     const int64_t im = max((int64_t)0, ii - 1), ip = min(n0 - 1, ii + 1);
@@ -89,8 +92,8 @@ __device__ __forceinline__ T half_sum(T a, T b) {
     return 0.5 * (a + b);
 }
 
-template <class T>
-__device__ __forceinline__ T tri_xy(const View<T>& c, int64_t a, int64_t b, int64_t cc, int pi, int pj) {
+template <class T, class V = View<T>>
+__device__ __forceinline__ T tri_xy(const V& c, int64_t a, int64_t b, int64_t cc, int pi, int pj) {
     if (!pi && !pj) return c.at(a, b, cc);
     if (pi && !pj) return half_sum<T>(c.at(a, b, cc), c.at(a + 1, b, cc));
     if (!pi && pj) return half_sum<T>(c.at(a, b, cc), c.at(a, b + 1, cc));
@@ -98,13 +101,13 @@ __device__ __forceinline__ T tri_xy(const View<T>& c, int64_t a, int64_t b, int6
                        half_sum<T>(c.at(a, b + 1, cc), c.at(a + 1, b + 1, cc)));
 }
 
-template <class T>
-__device__ __forceinline__ T trilinear(const View<T>& c, int64_t i, int64_t j, int64_t k) {
+template <class T, class V = View<T>>
+__device__ __forceinline__ T trilinear(const V& c, int64_t i, int64_t j, int64_t k) {
     const int pi = (int)(i & 1), pj = (int)(j & 1), pk = (int)(k & 1);
     const int64_t a = i >> 1, b = j >> 1, cc = k >> 1;
-    const T lo = tri_xy<T>(c, a, b, cc, pi, pj);
+    const T lo = tri_xy<T, V>(c, a, b, cc, pi, pj);
     if (!pk) return lo;
-    return half_sum<T>(lo, tri_xy<T>(c, a, b, cc + 1, pi, pj));
+    return half_sum<T>(lo, tri_xy<T, V>(c, a, b, cc + 1, pi, pj));
 }
 
 }  // namespace mgp

@@ -203,6 +203,10 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
         st.print_settings = !(c->flags & MGP_SOLVER_QUIET);
         if (c->refine_polish < 0) throw std::invalid_argument("refine_polish must be >= 0");
         st.refine_polish = c->refine_polish;
+        if (c->half_from < 0) throw std::invalid_argument("half_from must be >= 0");
+        if (c->half_from > 0 && c->dtype != MGP_MIXED)
+            throw std::invalid_argument("half_from needs dtype MGP_MIXED");
+        st.half_from = (Poisson::uint_t)c->half_from;
         auto t = [&](int q) { return c->bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
         Poisson::BoundaryCondition bc{t(MGP_NORTH), t(MGP_SOUTH), t(MGP_EAST), t(MGP_WEST), t(MGP_TOP), t(MGP_BOTTOM)};
         if (c->dtype == MGP_MIXED) {

@@ -42,7 +42,8 @@ class SolverConfig(ctypes.Structure):
                 ("origin", ctypes.c_double * 3), ("bc", ctypes.c_int32 * 6), ("dtype", ctypes.c_int32),
                 ("restriction", ctypes.c_int32), ("relaxation", ctypes.c_int32), ("mode", ctypes.c_int32),
                 ("flags", ctypes.c_int32), ("maxiter", ctypes.c_int64), ("miniter", ctypes.c_int64),
-                ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64), ("refine_polish", ctypes.c_int64)]
+                ("tolerance", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64), ("refine_polish", ctypes.c_int64),
+                ("half_from", ctypes.c_int64)]
 
 
 class BatchConfig(ctypes.Structure):
@@ -107,6 +108,13 @@ _SIGS = {
     "mgp_server_create": [P(ServerConfig), P(vp)], "mgp_server_destroy": [vp],
     "mgp_server_submit": [vp, P(Request), P(i64)], "mgp_server_wait": [vp, i64, P(Result)],
     "mgp_server_stats": [vp, P(i64), P(i64), P(i64)],
+    "mgp_fill_f16": [P(Array), d, vp], "mgp_rbgs_half_f16": [P(Array), P(Array), BOX, d, d, i32, vp],
+    "mgp_rbgs_sweeps_f16": [P(Array), P(Array), BOX, d, d, i64, vp],
+    "mgp_residual_f16": [P(Array), P(Array), P(Array), BOX, d, vp, vp],
+    "mgp_restrict_fw_scaled_f16": [P(Array), P(Array), vp, vp],
+    "mgp_restrict_fw_scaled_f32_f16": [P(Array), P(Array), vp, vp],
+    "mgp_prolong_add_scaled_f16": [P(Array), P(Array), vp, vp],
+    "mgp_prolong_add_scaled_f16_f32": [P(Array), P(Array), vp, vp],
 }
 for sfx in ("f64", "f32"):
     _SIGS.update({
@@ -202,7 +210,7 @@ class DeviceArray:
 
     @property
     def sfx(self):
-        return "f64" if self.dtype == np.float64 else "f32"
+        return {8: "f64", 4: "f32", 2: "f16"}[self.dtype.itemsize]
 
     def upload(self, host):
         h = np.ascontiguousarray(host, dtype=self.dtype).reshape(self.pshape)
@@ -267,7 +275,8 @@ class Solver:
 
     def __init__(self, dims, levels, h, origin=(0.0, 0.0, 0.0), bc="DDDDDD", mode=REFERENCE,
                  relaxation=GAUSS_SEIDEL, restriction=INJECTION, problem=0, dtype=F64, flags=QUIET,
-                 maxiter=100, miniter=1, tolerance=1e-6, coarse_sweeps=0, refine_polish=0):
+                 maxiter=100, miniter=1, tolerance=1e-6, coarse_sweeps=0, refine_polish=0,
+                 half_from=0):
         require_gpu()
         c = SolverConfig()
         c.dims[:] = [int(x) for x in dims]
@@ -280,6 +289,7 @@ class Solver:
         c.maxiter, c.miniter, c.tolerance = int(maxiter), int(miniter), float(tolerance)
         c.coarse_sweeps = int(coarse_sweeps)
         c.refine_polish = int(refine_polish)
+        c.half_from = int(half_from)
         self.config = c
         self.levels = int(levels)
         self.np_dtype = np.float32 if dtype == F32 else np.float64

@@ -28,6 +28,23 @@ class GaussSeidel;
 template <typename T>
 void residual(Domain<T>& domain, bool fetch_neighbor = true);
 
+// addition: the levels below a domain when another object keeps them (the
+// FP16 coarse levels of an FP32 cycle, fp16levels.h).  enter() forms the
+// domain's residual and restricts it into the first of them with a zero
+// guess; vcycle() / fcycle() run what the cycle templates run on those
+// levels (fcycle: the F-cycle's recursion below the domain, F where two or
+// more levels remain, then V); leave() adds the prolonged correction to the
+// domain's u.
+template <class T>
+class CoarseLevels {
+public:
+    virtual ~CoarseLevels() {}
+    virtual void enter(Domain<T>& d) = 0;
+    virtual void vcycle(T omega, int_t nsmooth) = 0;
+    virtual void fcycle(T omega, int_t nsmooth) = 0;
+    virtual void leave(Domain<T>& d) = 0;
+};
+
 template <class T>
 class Domain {
 private:
@@ -59,6 +76,9 @@ public:
     // addition: second device buffer of u for out-of-place red-black sweeps
     // (Gauss-Seidel with fused sweeps only; nullptr otherwise)
     T* u_alt = nullptr;
+    // addition: coarser levels kept elsewhere (the last level of a hierarchy
+    // only; nullptr otherwise)
+    CoarseLevels<T>* below = nullptr;
 
     Domain() : halo(), is_initialized(false), requires_duplicate(true), settings() {}
     Domain(Settings& _settings, BoundaryCondition& BC, bool requires_duplicate, int_t device[3], int_t device_shape[3]);

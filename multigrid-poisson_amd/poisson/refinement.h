@@ -11,6 +11,10 @@
 // restriction and prolongation; FP64 the residual, the update and the
 // convergence test, so the attainable accuracy is that of the FP64 residual.
 // It requires SolverMode CONSISTENT (homogeneous coarse Neumann data).
+//
+// Settings::half_from > 0 keeps levels half_from and coarser of the FP32
+// cycle in FP16 storage (Fp16Levels, fp16levels.h): the FP32 solver then has
+// half_from levels and its last one hands the cycle on to the FP16 levels.
 #ifndef POISSON_REFINEMENT
 #define POISSON_REFINEMENT
 
@@ -20,6 +24,9 @@
 #include <iostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
+
+#include "fp16levels.h"
 
 namespace Poisson {
 
@@ -34,6 +41,17 @@ public:
         if (s.mode != CONSISTENT)
             throw std::invalid_argument("RefinementSolver needs Settings::mode = CONSISTENT");
         outer_settings.levels = 1;
+        if (s.half_from > 0) {
+            if (s.half_from < 2 || s.half_from >= s.levels)
+                throw std::invalid_argument("RefinementSolver: half_from must lie in [2, levels)");
+            const bool dirichlet = bc.north == DIRICHLET && bc.south == DIRICHLET && bc.east == DIRICHLET &&
+                                   bc.west == DIRICHLET && bc.top == DIRICHLET && bc.bottom == DIRICHLET;
+            if (!dirichlet || !std::is_same<S<float>, GaussSeidel<float>>::value ||
+                !std::is_same<R<float>, FullWeighting<float>>::value)
+                throw std::invalid_argument(
+                    "RefinementSolver: FP16 levels need Gauss-Seidel, full weighting and Dirichlet faces");
+            inner_settings.levels = s.half_from;
+        }
         inner_settings.print_cycle_time = false;
         outer_settings.print_cycle_time = false;
         inner_settings.print_settings = false;  // the FP64 levels print, as one solver would
@@ -45,6 +63,18 @@ public:
             throw;
         }
         inner->use_stream(outer->get_stream());
+        if (s.half_from > 0) {
+            try {
+                mgpx::StreamScope scope(outer->get_stream());
+                Domain<float>& last = inner->get_domain(0, s.half_from - 1);
+                half = new Fp16Levels(last, s.levels - s.half_from, s.coarse_sweeps);
+                last.below = half;
+            } catch (...) {
+                delete inner;
+                delete outer;
+                throw;
+            }
+        }
         void* p = nullptr;
         mgpx::check(mgp_malloc(&p, 8 * sizeof(double)), "RefinementSolver");
         dev_norm = static_cast<double*>(p);
@@ -55,6 +85,7 @@ public:
     RefinementSolver& operator=(const RefinementSolver&) = delete;
     ~RefinementSolver() {
         delete inner;
+        delete half;
         delete outer;
         mgp_free(dev_norm);
         mgp_host_free(host_norm);
@@ -180,6 +211,7 @@ public:
     Boundary<double>& get_bottom_bc(uint_t g) { return outer->get_bottom_bc(g); }
     PoissonSolver<double, R, P, S>& fp64() { return *outer; }
     PoissonSolver<float, R, P, S>& fp32() { return *inner; }
+    Fp16Levels* fp16() { return half; }  // nullptr without FP16 levels
     void* get_stream() const { return outer->get_stream(); }
 
 private:
@@ -193,6 +225,7 @@ private:
     BoundaryCondition BC;
     PoissonSolver<double, R, P, S>* outer = nullptr;
     PoissonSolver<float, R, P, S>* inner = nullptr;
+    Fp16Levels* half = nullptr;
     double* dev_norm = nullptr;
     double* host_norm = nullptr;
     bool is_verbose = false;

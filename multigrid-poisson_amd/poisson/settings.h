@@ -57,6 +57,9 @@ public:
                                    // level (0 = nsmooth, the reference's schedule)
     int_t refine_polish = 0;       // RefinementSolver: FP64 smoothing passes on the finest level
                                    // after each update (0 = none)
+    uint_t half_from = 0;          // RefinementSolver: levels >= half_from of the FP32 cycle stored
+                                   // in FP16 (0 = none; Gauss-Seidel, full weighting, all faces
+                                   // Dirichlet)
 
     Settings() {}
     Settings(const Settings& s) = default;

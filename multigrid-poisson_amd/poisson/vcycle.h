@@ -58,9 +58,18 @@ template <class T>
 void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& prolongation,
             Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10) {
     if (level >= levels - 1) {
+        Domain<T>& d = *domains[0][level];
+        if (d.below) {  // coarser levels kept elsewhere: the same V-cycle through them
+            relaxation.relax(domains, omega, nsmooth, level, num_devices);
+            d.below->enter(d);
+            d.below->vcycle(omega, nsmooth);
+            d.below->leave(d);
+            relaxation.relax(domains, omega, nsmooth, level, num_devices);
+            return;
+        }
         // coarsest level: smoothing only (vcycle.h:27-31); in CONSISTENT mode
         // optionally more passes, a stronger coarsest-level solve
-        const Settings& st = domains[0][level]->settings;
+        const Settings& st = d.settings;
         const int_t n = (st.mode == CONSISTENT && st.coarse_sweeps > 0) ? st.coarse_sweeps : nsmooth;
         relaxation.relax(domains, omega, n, level, num_devices);
         return;

@@ -690,3 +690,169 @@ double* og_face(og_solver* s, int level, int loc) { return s->lv[level].g[loc]; 
 const og_layout* og_face_layout(og_solver* s, int level, int loc) { return &s->lv[level].GL[loc]; }
 int32_t og_face_type(og_solver* s, int level, int loc) { return s->lv[level].type[loc]; }
 double og_level_h(og_solver* s, int level) { return s->lv[level].h; }
+
+/* ------------------------------------------ FP16 storage, FP32 arithmetic */
+
+/* binary16 bits -> float (exact) */
+float og_h2f(uint16_t h) {
+    const uint32_t sign = (uint32_t)(h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 31u, m = h & 1023u;
+    uint32_t b;
+    if (e == 31) {
+        b = sign | 0x7f800000u | (m << 13);
+    } else if (e == 0) {
+        float v = ldexpf((float)m, -24); /* subnormal (or zero): m 2^-24, exact */
+        memcpy(&b, &v, 4);
+        b |= sign;
+    } else {
+        b = sign | ((e + 112u) << 23) | (m << 13); /* exponent bias 15 -> 127 */
+    }
+    float out;
+    memcpy(&out, &b, 4);
+    return out;
+}
+
+/* float -> binary16, round to nearest, ties to even */
+uint16_t og_f2h(float x) {
+    uint32_t b;
+    memcpy(&b, &x, 4);
+    const uint16_t sign = (uint16_t)((b >> 16) & 0x8000u);
+    const uint32_t a = b & 0x7fffffffu;
+    if (a > 0x7f800000u) return (uint16_t)(sign | 0x7e00u);           /* NaN */
+    if (a >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);          /* >= 65520 rounds to infinity */
+    if (a < 0x38800000u) {                                            /* below 2^-14: subnormal result */
+        float v;
+        memcpy(&v, &a, 4);
+        return (uint16_t)(sign | (uint16_t)nearbyintf(v * 16777216.0f)); /* v 2^24 is exact; RNE to an integer */
+    }
+    const uint32_t e = (a >> 23) - 112u; /* 1..30 */
+    const uint32_t m = a & 0x7fffffu;
+    uint32_t hb = (e << 10) | (m >> 13);
+    const uint32_t rem = m & 0x1fffu;
+    if (rem > 0x1000u || (rem == 0x1000u && (hb & 1u))) hb++; /* a carry moves into the exponent */
+    return (uint16_t)(sign | hb);
+}
+
+float og_h_scale(double norm, int inverse) {
+    if (!(norm > 0.0) || !(norm < 1e300)) return 1.0f;
+    int e;
+    (void)frexp(norm, &e);
+    e = MIN(120, MAX(-120, e));
+    return ldexpf(1.0f, inverse ? e : -e);
+}
+
+#define HV(a, c) og_h2f((a)[c])
+
+/* consistent-mode gs_point on binary16 storage; the constants as the FP32
+ * smoother forms them from (float)omega and h */
+void og_h_gs_half(const og_layout* L, uint16_t* u, const uint16_t* f, const int64_t box[6], double omega, double h,
+                  int colour) {
+    const float omega_sixth = (float)((double)(float)omega / 6.0);
+    const float hsq = (float)(h * h);
+    const int64_t s0 = L->s[0], s1 = L->s[1];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = box[0]; i < box[1]; i++)
+        for (int64_t j = box[2]; j < box[3]; j++)
+            for (int64_t k = box[4]; k < box[5]; k++) {
+                if ((int)((i + j + k) & 1) != colour) continue;
+                const int64_t c = IX(L, i, j, k);
+                const float uc = HV(u, c);
+                const float s = ((HV(u, c - s0) - uc) + (HV(u, c + s0) - uc)) +
+                                ((HV(u, c - s1) - uc) + (HV(u, c + s1) - uc)) + ((HV(u, c - 1) - uc) + (HV(u, c + 1) - uc));
+                u[c] = og_f2h(fmaf(omega_sixth, fmaf(-hsq, HV(f, c), s), uc));
+            }
+}
+
+double og_h_residual(const og_layout* L, const uint16_t* u, const uint16_t* f, uint16_t* r, const int64_t box[6],
+                     double h) {
+    const float inv_h2 = (float)(1.0 / (h * h));
+    const int64_t s0 = L->s[0], s1 = L->s[1];
+#pragma omp parallel for schedule(static)
+    for (int64_t i = box[0]; i < box[1]; i++)
+        for (int64_t j = box[2]; j < box[3]; j++)
+            for (int64_t k = box[4]; k < box[5]; k++) {
+                const int64_t c = IX(L, i, j, k);
+                const float uc = HV(u, c);
+                const float s = ((HV(u, c - s0) - uc) + (HV(u, c + s0) - uc)) +
+                                ((HV(u, c - s1) - uc) + (HV(u, c + s1) - uc)) + ((HV(u, c - 1) - uc) + (HV(u, c + 1) - uc));
+                r[c] = og_f2h(fmaf(-inv_h2, s, HV(f, c)));
+            }
+    double m = 0.0;
+    for (int64_t i = 0; i < L->n[0]; i++)
+        for (int64_t j = 0; j < L->n[1]; j++)
+            for (int64_t k = 0; k < L->n[2]; k++) m = MAX(m, fabs((double)HV(r, IX(L, i, j, k))));
+    return m;
+}
+
+/* og_restrict_fw's weights and term order with the running sum held in
+ * float between the three planes, as the FP32 kernels keep it */
+void og_h_restrict_fw(const og_layout* Li, const void* in, int in_f32, const og_layout* Lo, uint16_t* out,
+                      double norm) {
+    const int64_t n0 = Li->n[0], n1 = Li->n[1], n2 = Li->n[2];
+    const float sigma = og_h_scale(norm, 0);
+    const float* in32 = (const float*)in;
+    const uint16_t* in16 = (const uint16_t*)in;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < Lo->n[0]; i++)
+        for (int64_t j = 0; j < Lo->n[1]; j++)
+            for (int64_t k = 0; k < Lo->n[2]; k++) {
+                const int64_t ii = MIN(n0 - 1, 2 * i), jj = MIN(n1 - 1, 2 * j), kk = MIN(n2 - 1, 2 * k);
+                const int64_t im = MAX(0, ii - 1), ip = MIN(n0 - 1, ii + 1);
+                const int64_t jm = MAX(0, jj - 1), jp = MIN(n1 - 1, jj + 1);
+                const int64_t km = MAX(0, kk - 1), kp = MIN(n2 - 1, kk + 1);
+#define A(a, b, c) (in_f32 ? in32[IX(Li, a, b, c)] : og_h2f(in16[IX(Li, a, b, c)]))
+                float tmp = (float)(1.0 * A(im, jm, km) + 2.0 * A(im, jj, km) + 1.0 * A(im, jp, km) +
+                                    2.0 * A(ii, jm, km) + 4.0 * A(ii, jj, km) + 2.0 * A(ii, jp, km) +
+                                    1.0 * A(ip, jm, km) + 2.0 * A(ip, jj, km) + 1.0 * A(ip, jp, km));
+                tmp = (float)((double)tmp + (2.0 * A(im, jm, kk) + 4.0 * A(im, jj, kk) + 2.0 * A(im, jp, kk) +
+                                             4.0 * A(ii, jm, kk) + 8.0 * A(ii, jj, kk) + 4.0 * A(ii, jp, kk) +
+                                             2.0 * A(ip, jm, kk) + 4.0 * A(ip, jj, kk) + 2.0 * A(ip, jp, kk)));
+                tmp = (float)((double)tmp + (1.0 * A(im, jm, kp) + 2.0 * A(im, jj, kp) + 1.0 * A(im, jp, kp) +
+                                             2.0 * A(ii, jm, kp) + 4.0 * A(ii, jj, kp) + 2.0 * A(ii, jp, kp) +
+                                             1.0 * A(ip, jm, kp) + 2.0 * A(ip, jj, kp) + 1.0 * A(ip, jp, kp)));
+#undef A
+                const float v = (float)((double)tmp / 64.0);
+                out[IX(Lo, i, j, k)] = og_f2h(v * sigma);
+            }
+}
+
+/* 0.5*(a+b): the float sum, halved */
+static inline float half_sum_f(float a, float b) { return (float)(0.5 * (double)(a + b)); }
+
+/* trilinear value at fine (i,j,k) from the coarse values around it,
+ * composed as og_prolong_trilinear's x, y and z passes compose it */
+static float tri_xy_f(const og_layout* Lc, const uint16_t* c, int64_t a, int64_t b, int64_t cc, int pi, int pj) {
+#define C(x, y, z) og_h2f(c[IX(Lc, x, y, z)])
+    if (!pi && !pj) return C(a, b, cc);
+    if (pi && !pj) return half_sum_f(C(a, b, cc), C(a + 1, b, cc));
+    if (!pi && pj) return half_sum_f(C(a, b, cc), C(a, b + 1, cc));
+    return half_sum_f(half_sum_f(C(a, b, cc), C(a + 1, b, cc)), half_sum_f(C(a, b + 1, cc), C(a + 1, b + 1, cc)));
+#undef C
+}
+
+void og_h_prolong_add(const og_layout* Lc, const uint16_t* coarse, const og_layout* Lf, void* fine, int fine_f32,
+                      double norm) {
+    const float inv = og_h_scale(norm, 1);
+    float* f32 = (float*)fine;
+    uint16_t* f16 = (uint16_t*)fine;
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < Lf->n[0]; i++)
+        for (int64_t j = 0; j < Lf->n[1]; j++)
+            for (int64_t k = 0; k < Lf->n[2]; k++) {
+                const int pi = (int)(i & 1), pj = (int)(j & 1), pk = (int)(k & 1);
+                const float lo = tri_xy_f(Lc, coarse, i >> 1, j >> 1, k >> 1, pi, pj);
+                const float v = (pk ? half_sum_f(lo, tri_xy_f(Lc, coarse, i >> 1, j >> 1, (k >> 1) + 1, pi, pj)) : lo) *
+                                inv;
+                const int64_t c = IX(Lf, i, j, k);
+                if (fine_f32) f32[c] = f32[c] + v;
+                else f16[c] = og_f2h(og_h2f(f16[c]) + v);
+            }
+}
+#undef HV
+
+void og_f2h_n(const float* x, uint16_t* out, int64_t n) {
+    for (int64_t t = 0; t < n; t++) out[t] = og_f2h(x[t]);
+}
+void og_h2f_n(const uint16_t* x, float* out, int64_t n) {
+    for (int64_t t = 0; t < n; t++) out[t] = og_h2f(x[t]);
+}

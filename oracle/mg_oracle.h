@@ -117,6 +117,35 @@ const og_layout* og_face_layout(og_solver* s, int level, int loc);
 int32_t og_face_type(og_solver* s, int level, int loc);
 double og_level_h(og_solver* s, int level);
 
+/* ---- FP16 storage, FP32 arithmetic (the coarse levels of an FP32 cycle;
+ * this project's addition, the reference has one type T throughout).
+ * Arrays hold IEEE binary16 bit patterns; values are loaded as float,
+ * combined in float with the consistent-mode forms (fmaf), and rounded to
+ * binary16 (nearest, ties to even) when stored.  Consistent mode only:
+ * colour = parity of i+j+k, boxes strictly inside the shape.
+ * Pinned by: og_f2h/og_h2f against numpy's binary16 conversion (every
+ * binary16 value, every midpoint between neighbours, random floats); each
+ * operator against the FP64 operator above on inputs where every float
+ * operation is exact (so the two must agree after rounding), and against it
+ * within the binary16 rounding bound on random inputs. */
+uint16_t og_f2h(float x);
+float og_h2f(uint16_t h);
+void og_f2h_n(const float* x, uint16_t* out, int64_t n);
+void og_h2f_n(const uint16_t* x, float* out, int64_t n);
+/* 2^-e (inverse: 2^e) for norm = m 2^e, m in [0.5,1); 1 for norm <= 0 */
+float og_h_scale(double norm, int inverse);
+void og_h_gs_half(const og_layout* L, uint16_t* u, const uint16_t* f, const int64_t box[6], double omega, double h,
+                  int colour);
+/* r = f - L u on the box; returns max |r| of the stored r over its shape */
+double og_h_residual(const og_layout* L, const uint16_t* u, const uint16_t* f, uint16_t* r, const int64_t box[6],
+                     double h);
+/* out = sigma(norm) * FW(in), in binary16 (in_f32 = 0) or float (in_f32 = 1) */
+void og_h_restrict_fw(const og_layout* Li, const void* in, int in_f32, const og_layout* Lo, uint16_t* out,
+                      double norm);
+/* fine += P(coarse) / sigma(norm) over the fine shape, fine binary16 or float */
+void og_h_prolong_add(const og_layout* Lc, const uint16_t* coarse, const og_layout* Lf, void* fine, int fine_f32,
+                      double norm);
+
 #ifdef __cplusplus
 }
 #endif

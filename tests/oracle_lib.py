@@ -80,6 +80,15 @@ def lib():
         L.og_face_type.restype = ctypes.c_int32
         L.og_level_h.argtypes = [vp, i32]
         L.og_level_h.restype = d
+        L.og_f2h_n.argtypes = [vp, vp, i64]
+        L.og_h2f_n.argtypes = [vp, vp, i64]
+        L.og_h_scale.argtypes = [d, i32]
+        L.og_h_scale.restype = ctypes.c_float
+        L.og_h_gs_half.argtypes = [P(Layout), vp, vp, P(i64 * 6), d, d, i32]
+        L.og_h_residual.argtypes = [P(Layout), vp, vp, vp, P(i64 * 6), d]
+        L.og_h_residual.restype = d
+        L.og_h_restrict_fw.argtypes = [P(Layout), vp, i32, P(Layout), vp, d]
+        L.og_h_prolong_add.argtypes = [P(Layout), vp, P(Layout), vp, i32, d]
         _lib = L
     return _lib
 
@@ -187,3 +196,55 @@ class Solver:
 def cycle_id(name):
     n = name.lower()
     return {"vcycle": VCYCLE, "fcycle": FCYCLE, "relaxation": RELAXATION}[n]
+
+
+# ---- FP16 storage, FP32 arithmetic (arrays without halos: numpy float16,
+# or float32 where a function takes FP32 input or output)
+
+def _hp(a, dtypes=(np.float16,)):
+    assert a.dtype in dtypes and a.flags.c_contiguous, a.dtype
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def f2h(x):
+    """float32 -> float16 by the oracle's own rounding (nearest, ties to even)."""
+    x = np.ascontiguousarray(x, dtype=np.float32)
+    out = np.empty(x.shape, dtype=np.uint16)
+    lib().og_f2h_n(x.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), x.size)
+    return out.view(np.float16)
+
+
+def h2f(x):
+    x = np.ascontiguousarray(x, dtype=np.float16)
+    out = np.empty(x.shape, dtype=np.float32)
+    lib().og_h2f_n(x.ctypes.data_as(ctypes.c_void_p), out.ctypes.data_as(ctypes.c_void_p), x.size)
+    return out
+
+
+def h_scale(norm, inverse=False):
+    return float(lib().og_h_scale(float(norm), int(inverse)))
+
+
+def h_gs_half(u, f, box, omega, h, colour):
+    """In place on float16 u."""
+    lib().og_h_gs_half(ctypes.byref(layout(u.shape)), _hp(u), _hp(f), ctypes.byref(box_arr(box)), float(omega),
+                       float(h), int(colour))
+
+
+def h_residual(u, f, r, box, h):
+    """r (float16) = f - L u on the box; returns max |r| over r's shape."""
+    return lib().og_h_residual(ctypes.byref(layout(u.shape)), _hp(u), _hp(f), _hp(r), ctypes.byref(box_arr(box)),
+                               float(h))
+
+
+def h_restrict_fw(fine, coarse, norm):
+    """coarse (float16) = sigma(norm) FW(fine), fine float16 or float32."""
+    lib().og_h_restrict_fw(ctypes.byref(layout(fine.shape)), _hp(fine, (np.float16, np.float32)),
+                           int(fine.dtype == np.float32), ctypes.byref(layout(coarse.shape)), _hp(coarse),
+                           float(norm))
+
+
+def h_prolong_add(coarse, fine, norm):
+    """fine (float16 or float32) += P(coarse) / sigma(norm), in place."""
+    lib().og_h_prolong_add(ctypes.byref(layout(coarse.shape)), _hp(coarse), ctypes.byref(layout(fine.shape)),
+                           _hp(fine, (np.float16, np.float32)), int(fine.dtype == np.float32), float(norm))
