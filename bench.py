@@ -17,7 +17,8 @@ The mixed-precision leg (same grid and cycle, all-Dirichlet faces,
 u = cos(x z^2) sin(y^3)) times the FP32-cycle / FP64-refinement solver and the
 plain FP64 solver on the same problem, and reports their DOF/s, the ratio, and
 the max-abs difference of the mixed solution from the FP64 one and from the
-analytic u (--no-mixed skips it).
+analytic u; the same with levels >= 3 of the FP32 cycle stored in FP16
+(--no-mixed skips it).
 
 Prints one JSON line (see the keys below).  --gpus N > 1 exits: this build
 runs on one GPU.
@@ -247,9 +248,10 @@ def interior(a, shape, halo):
 def mixed_leg(dims, h, args):
     """FP32 cycles inside FP64 refinement vs plain FP64, same grid, levels,
     F-cycle and coarsest passes.  The all-Dirichlet problem
-    u = cos(x z^2) sin(y^3) at tol 1e-10 and at 2e-10, and the headline
-    problem (Neumann x/y, u = x^3 y^2 z) at 1e-10; max-abs differences of the
-    all-Dirichlet solutions from each other and from the analytic u."""
+    u = cos(x z^2) sin(y^3) at tol 1e-10 and at 2e-10 (also with levels >= 3
+    of the FP32 cycle stored in FP16), and the Neumann-x/y problem
+    u = x^3 y^2 z at 1e-10; max-abs differences of the all-Dirichlet
+    solutions from each other and from the analytic u."""
     import numpy as np
 
     dof = dims[0] * dims[1] * dims[2]
@@ -260,12 +262,15 @@ def mixed_leg(dims, h, args):
     for label, prob, bc, tol in cases:
         key = "%s_tol%.0e" % (label, tol)
         out[key] = {}
-        for name, dt in (("fp64", mgp.F64), ("fp32_refine", mgp.MIXED)):
+        variants = [("fp64", mgp.F64, 0), ("fp32_refine", mgp.MIXED, 0)]
+        if bc == "DDDDDD":  # FP16 storage of levels >= 3 (Dirichlet faces only)
+            variants.append(("fp32_refine_fp16_coarse", mgp.MIXED, 3))
+        for name, dt, half_from in variants:
             log("bench: mixed leg, %s %s" % (key, name))
             s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
                            mgp.FULL_WEIGHTING, problem=prob, dtype=dt,
                            flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0), maxiter=30, miniter=1,
-                           tolerance=tol, coarse_sweeps=args.coarse_sweeps)
+                           tolerance=tol, coarse_sweeps=args.coarse_sweeps, half_from=half_from)
             s.to_device()
             el, cyc, rel = timed_solves(s, args)
             out[key][name] = {"DOF/s": dof * args.steps / el, "ms_per_solve": 1e3 * el / args.steps,
@@ -278,22 +283,28 @@ def mixed_leg(dims, h, args):
             s.close()
         o = out[key]
         o["speedup"] = o["fp32_refine"]["DOF/s"] / o["fp64"]["DOF/s"]
-        o["ms_per_iteration"] = {n: o[n]["ms_per_solve"] / max(o[n]["iterations"], 1) for n in ("fp64", "fp32_refine")}
-    a, b = sols["fp32_refine"], sols["fp64"]
+        o["ms_per_iteration"] = {n: o[n]["ms_per_solve"] / max(o[n]["iterations"], 1) for n, _, _ in variants}
+        if len(variants) > 2:
+            o["speedup_fp16_coarse"] = o["fp32_refine_fp16_coarse"]["DOF/s"] / o["fp64"]["DOF/s"]
+    a, b, c = sols["fp32_refine"], sols["fp64"], sols["fp32_refine_fp16_coarse"]
     x = np.arange(dims[0]) * h
     y = np.arange(dims[1]) * h
     z = np.arange(dims[2]) * h
     sy3 = np.sin(y ** 3)[None, :, None]
-    d_fp64 = err_mixed = err_fp64 = 0.0
+    d_fp64 = d16_fp64 = err_mixed = err_fp64 = err16 = 0.0
     for i0 in range(0, dims[0], 64):
         xs = x[i0:i0 + 64][:, None, None]
         ua = np.cos(xs * (z ** 2)[None, None, :]) * sy3
         d_fp64 = max(d_fp64, float(np.max(np.abs(a[i0:i0 + 64] - b[i0:i0 + 64]))))
+        d16_fp64 = max(d16_fp64, float(np.max(np.abs(c[i0:i0 + 64] - b[i0:i0 + 64]))))
         err_mixed = max(err_mixed, float(np.max(np.abs(a[i0:i0 + 64] - ua))))
         err_fp64 = max(err_fp64, float(np.max(np.abs(b[i0:i0 + 64] - ua))))
-    out["cos_dirichlet_max_abs_diff_vs_fp64"] = d_fp64
-    out["cos_dirichlet_max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp64": err_fp64}
-    log("bench: mixed |u_mixed - u_fp64| %.2e, errors vs analytic %.3e / %.3e" % (d_fp64, err_mixed, err_fp64))
+        err16 = max(err16, float(np.max(np.abs(c[i0:i0 + 64] - ua))))
+    out["cos_dirichlet_max_abs_diff_vs_fp64"] = {"fp32_refine": d_fp64, "fp32_refine_fp16_coarse": d16_fp64}
+    out["cos_dirichlet_max_abs_err_vs_analytic"] = {"fp32_refine": err_mixed, "fp32_refine_fp16_coarse": err16,
+                                                    "fp64": err_fp64}
+    log("bench: mixed |u_mixed - u_fp64| %.2e (FP16 coarse %.2e), errors vs analytic %.3e / %.3e / %.3e"
+        % (d_fp64, d16_fp64, err_mixed, err16, err_fp64))
     return out
 
 
