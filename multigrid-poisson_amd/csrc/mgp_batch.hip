@@ -92,6 +92,27 @@ __device__ __forceinline__ double wave_max64(double v) {
     return v;
 }
 
+// Workgroups are dispatched round-robin over the 8 XCDs (blockIdx % 8), each
+// with its own L2.  Neighbouring items of a list share halo rows, so each XCD
+// takes one contiguous eighth of the list: its workgroups walk that range
+// with a stride of their own count, and the items in flight on one XCD are
+// neighbours.  Grids not a multiple of 8 walk the whole list round-robin.
+struct Walk {
+    int first, end, step;
+};
+__device__ __forceinline__ Walk xcd_walk(int nitems) {
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    if (G % 8) return Walk{b, nitems, G};
+    const int per = G / 8, x = b % 8;
+    const int lo = (int)((int64_t)nitems * x / 8), hi = (int)((int64_t)nitems * (x + 1) / 8);
+    return Walk{lo + b / 8, hi, per};
+}
+// this workgroup's place when workgroups are numbered XCD by XCD
+__device__ __forceinline__ int xcd_rank() {
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    return G % 8 ? b : (b % 8) * (G / 8) + b / 8;
+}
+
 // max |.| of the FP64 residual or of f per problem: each workgroup walks a
 // contiguous run of items (the lists hold each problem's items together) and
 // keeps the running max of the problem at hand, folding it into the
@@ -100,7 +121,7 @@ __device__ __forceinline__ double wave_max64(double v) {
 template <int OP>
 __device__ __forceinline__ void batch_norms(const BArgs& a) {
     const int per = (a.nitems + (int)gridDim.x - 1) / (int)gridDim.x;
-    const int i0 = (int)blockIdx.x * per, i1 = min(a.nitems, i0 + per);
+    const int i0 = xcd_rank() * per, i1 = min(a.nitems, i0 + per);
     int cur = -1;
     double vmax = 0.0;
     auto flush = [&]() {
@@ -140,13 +161,58 @@ __device__ __forceinline__ void batch_norms(const BArgs& a) {
     if (cur >= 0) flush();
 }
 
+// Jacobi sweep or residual of one item's column k: march along x holding
+// u(i-1), u(i), u(i+1) in registers, so each point loads one new x-plane value
+// instead of three (and the restrict-qualified pointers let the next point's
+// loads issue before this point's store)
+template <int OP>
+__device__ __forceinline__ void march_item(const BGrid& g, const BItem& item, int src, int64_t k) {
+    const View<float>& v = g.u[src];
+    const View<float>& w = OP == OP_JACOBI ? g.u[1 - src] : g.r;
+    const float* __restrict__ q = v.p;
+    const float* __restrict__ fp = g.f.p;
+    float* __restrict__ o = w.p;
+    const int64_t s0 = v.s0, s1 = v.s1;
+    for (int64_t j = item.y0 + threadIdx.y; j < item.y1; j += blockDim.y) {
+        int64_t c = v.ix(item.x0, j, k), cf = g.f.ix(item.x0, j, k), co = w.ix(item.x0, j, k);
+        float um = q[c - s0], uc = q[c];
+        for (int64_t i = item.x0; i < item.x1; i++) {
+            const float up = q[c + s0];
+            const float uS = q[c - s1], uN = q[c + s1], uB = q[c - 1], uT = q[c + 1], fc = fp[cf];
+            if (OP == OP_JACOBI) {
+                o[co] = relax_point<float, MGP_MODE_CONSISTENT>(uc, um, up, uS, uN, uB, uT, fc, g.one_omega,
+                                                                g.omega_sixth, g.hsq);
+            } else {  // residual_point's consistent form
+                const float sum = ((um - uc) + (up - uc)) + ((uS - uc) + (uN - uc)) + ((uB - uc) + (uT - uc));
+                o[co] = fma(-g.inv_h2, sum, fc);
+            }
+            um = uc;
+            uc = up;
+            c += s0;
+            cf += g.f.s0;
+            co += w.s0;
+        }
+    }
+}
+
 template <int OP>
 __global__ void __launch_bounds__(256) k_batch(BArgs a) {
     if (OP == OP_RESIDUAL64 || OP == OP_FNORM) {
         batch_norms<OP>(a);
         return;
     }
-    for (int it = blockIdx.x; it < a.nitems; it += gridDim.x) {
+    const Walk w = xcd_walk(a.nitems);
+    if (OP == OP_JACOBI || OP == OP_RESIDUAL) {
+        for (int it = w.first; it < w.end; it += w.step) {
+            const BItem item = a.items[it];
+            const BGrid& g = a.grids[item.grid];
+            if (!a.state[g.problem].active) continue;  // uniform per workgroup
+            const int64_t k = item.z0 + threadIdx.x;
+            if (k >= 1 && k < g.u[a.src].n2 - 1) march_item<OP>(g, item, a.src, k);
+        }
+        return;
+    }
+    for (int it = w.first; it < w.end; it += w.step) {
         const BItem item = a.items[it];
         const BGrid& g = a.grids[item.grid];
         const int p = g.problem;
@@ -155,21 +221,12 @@ __global__ void __launch_bounds__(256) k_batch(BArgs a) {
         const View<float>& us = g.u[a.src];
         const int64_t kz1 = us.n2;  // level 0 of the FP64 arrays has the level-0 shape
         // interior-only operations stop one short of the far face
-        constexpr bool interior = OP == OP_JACOBI || OP == OP_GS_HALF || OP == OP_RESIDUAL;
+        constexpr bool interior = OP == OP_GS_HALF;  // Jacobi and the residual: march_item
         const int64_t kmax = interior ? kz1 - 1 : kz1;
         if (k < kmax && (!interior || k >= 1)) {
             for (int64_t j = item.y0 + threadIdx.y; j < item.y1; j += blockDim.y) {
                 for (int64_t i = item.x0; i < item.x1; i++) {
                     switch (OP) {
-                    case OP_JACOBI: {
-                        const View<float>& v = us;
-                        const int64_t c = v.ix(i, j, k);
-                        const float* q = v.p;
-                        g.u[1 - a.src].at(i, j, k) = relax_point<float, MGP_MODE_CONSISTENT>(
-                            q[c], q[c - v.s0], q[c + v.s0], q[c - v.s1], q[c + v.s1], q[c - 1], q[c + 1],
-                            g.f.at(i, j, k), g.one_omega, g.omega_sixth, g.hsq);
-                        break;
-                    }
                     case OP_GS_HALF: {
                         if ((int)((i + j + k) & 1) != a.colour) break;
                         const int64_t c = us.ix(i, j, k);
@@ -180,10 +237,6 @@ __global__ void __launch_bounds__(256) k_batch(BArgs a) {
                                                                        g.hsq);
                         break;
                     }
-                    case OP_RESIDUAL:
-                        g.r.at(i, j, k) = residual_point<float, MGP_MODE_CONSISTENT>(us, us.ix(i, j, k),
-                                                                                     g.f.at(i, j, k), g.hsq, g.inv_h2);
-                        break;
                     case OP_FW: {
                         const BGrid& fg = a.grids[g.fine];
                         g.f.at(i, j, k) = fw_point<float>(fg.r, i, j, k);
@@ -538,7 +591,8 @@ private:
     void autotune() {
         // candidate tiles, timed per distinct level shape on a Jacobi sweep of
         // every grid of that shape
-        static const Tile cands[] = {{1, 4}, {2, 4}, {4, 4}, {8, 4}, {1, 8}, {2, 8}, {4, 8}, {1, 16}, {4, 16}};
+        static const Tile cands[] = {{1, 4},  {2, 4}, {4, 4}, {8, 4},  {16, 4}, {1, 8},
+                                     {2, 8}, {4, 8}, {8, 8}, {1, 16}, {4, 16}};
         const int ng = (int)gshape.size();
         tiles.assign(ng, Tile{1, 4});
         std::map<std::tuple<int64_t, int64_t, int64_t>, std::vector<int>> by_shape;
@@ -643,7 +697,8 @@ private:
     }
 
     static int persistent_blocks(int nitems) {
-        return (int)std::max<int64_t>(1, std::min<int64_t>(nitems, g_batch_blocks));
+        const int64_t g = std::max<int64_t>(1, std::min<int64_t>(nitems, g_batch_blocks));
+        return (int)(g >= 64 ? (g + 7) / 8 * 8 : g);  // a multiple of 8 walks XCD by XCD
     }
 
     BArgs args(int op, const BItem* items, int nitems, int src, int src2, int colour) const {

@@ -91,7 +91,7 @@ def test_batch_equals_single_solves_bitwise(relaxation, nsmooth, omega, cs, grap
 
 def test_batch_tiles_come_from_the_candidates():
     b = mgp.Batch([(33, 33, 33), (65, 33, 17)], maxiter=1)
-    cands = {(1, 4), (2, 4), (4, 4), (8, 4), (1, 8), (2, 8), (4, 8), (1, 16), (4, 16)}
+    cands = {(1, 4), (2, 4), (4, 4), (8, 4), (16, 4), (1, 8), (2, 8), (4, 8), (8, 8), (1, 16), (4, 16)}
     for p, d in enumerate([(33, 33, 33), (65, 33, 17)]):
         for level in range(max_levels(d)):
             assert b.tile(p, level) in cands
