@@ -1823,6 +1823,7 @@ int64_t g_fw_rows = 0;   // coarse rows per residual + full-weighting wave tile:
 int64_t g_fw_depth = 0;  // fine planes of loads in flight per wave: 1 or 2; 0: 1 (FP64), 2 (FP32)
 
 // Launch-shape knobs of the one-pass sweep (set through mgp_tuning_set).
+int64_t g_sweep2_cpt = 15;          // two-sweep pass: least x chunks per (y, z) tile
 int64_t g_sweep_target_wgs = 16384;  // workgroups (waves, kernel 2) to aim for per launch
 int64_t g_sweep_min_chunk = 4;      // shortest x march per workgroup / wave: coarse levels are
                                     // latency-bound, so they get many short marches
@@ -1876,12 +1877,14 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
     const int64_t nx = b[1] - b[0];
-    // each chunk spends 6 warm-up steps: aim at 16 chunks per tile, between
-    // 4096 waves and the sweep target (measured at 1025 x 1025 x 513 and its
-    // coarser levels: 16384 waves on the finest, ~5000 on 513 x 513 x 257,
-    // 4096 below)
-    const int64_t target = std::min<int64_t>(g_sweep_target_wgs, std::max<int64_t>(4096, 16 * tiles));
-    int64_t chunks = std::max<int64_t>(1, target / std::max<int64_t>(tiles, 1));
+    // x chunks per (y, z) tile (each chunk spends 6 warm-up steps): at least
+    // sweep2.chunks_per_tile, enough for about 6800 waves on coarser levels,
+    // and an odd count.  Measured, FP64: 1025 x 1025 x 513 with 13, 15, 17
+    // chunks 4.94-5.01 ms, with 12, 14, 16, 18 chunks 5.19-5.21 ms;
+    // 513 x 513 x 257 with 21 chunks 0.82 ms, with 16 0.89 ms; 257 x 257 x 129
+    // and below flat.
+    const int64_t tl = std::max<int64_t>(tiles, 1);
+    int64_t chunks = std::max<int64_t>(g_sweep2_cpt, (6800 + tl - 1) / tl) | 1;
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
     A.xchunk = (nx + chunks - 1) / chunks;
     A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
