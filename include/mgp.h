@@ -473,7 +473,11 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * default; 0: one pass per sweep), "sweep2.rows" (register rows per two-sweep
  * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16, default),
  * "sweep2.chunks_per_tile" (least x chunks per two-sweep wave tile, made odd,
- * and at least enough for about 6800 waves; default 15), "residual.kernel" (1:
+ * and at least enough for about 6800 waves; default 15), "residual.target_wgs",
+ * "residual_fw.target_wgs", "prolong.target_wgs" (waves the norm-only
+ * residual, the fused residual + full-weighting and the prolongation marches
+ * aim for; 0 = the value of
+ * "sweep.target_wgs"; defaults 4096, 32768, 65536), "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
  * "residual.rows" (8 or 16 register rows per residual tile, default 16),
  * "residual_fw.rows" (4 or 6 coarse rows per wave tile of the fused residual

@@ -121,6 +121,9 @@ int mgp_tuning_set(const char* key, int64_t value) {
     if (!strcmp(key, "sweep.target_wgs") && value >= 1) mgp::g_sweep_target_wgs = value;
     else if (!strcmp(key, "sweep.min_chunk") && value >= 1) mgp::g_sweep_min_chunk = value;
     else if (!strcmp(key, "sweep2.chunks_per_tile") && value >= 1 && value <= 1024) mgp::g_sweep2_cpt = value;
+    else if (!strcmp(key, "residual.target_wgs") && value <= (1 << 20)) mgp::g_resid_target = value;
+    else if (!strcmp(key, "residual_fw.target_wgs") && value <= (1 << 20)) mgp::g_fw_target = value;
+    else if (!strcmp(key, "prolong.target_wgs") && value <= (1 << 20)) mgp::g_prolong_target = value;
     else if (!strcmp(key, "sweep.waves") && (value == 4 || value == 8 || value == 16)) mgp::g_sweep_waves = value;
     else if (!strcmp(key, "sweep.kernel") && (value == 1 || value == 2)) mgp::g_sweep_kernel = value;
     else if (!strcmp(key, "sweep.rows") && (value == 0 || value == 8 || value == 12 || value == 16))
@@ -145,6 +148,9 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     if (!strcmp(key, "sweep.target_wgs")) *value = mgp::g_sweep_target_wgs;
     else if (!strcmp(key, "sweep.min_chunk")) *value = mgp::g_sweep_min_chunk;
     else if (!strcmp(key, "sweep2.chunks_per_tile")) *value = mgp::g_sweep2_cpt;
+    else if (!strcmp(key, "residual.target_wgs")) *value = mgp::g_resid_target;
+    else if (!strcmp(key, "residual_fw.target_wgs")) *value = mgp::g_fw_target;
+    else if (!strcmp(key, "prolong.target_wgs")) *value = mgp::g_prolong_target;
     else if (!strcmp(key, "sweep.waves")) *value = mgp::g_sweep_waves;
     else if (!strcmp(key, "sweep.kernel")) *value = mgp::g_sweep_kernel;
     else if (!strcmp(key, "sweep.rows")) *value = mgp::g_sweep_rows;
