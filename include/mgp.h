@@ -470,7 +470,11 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * 0 = 12 for FP64 and 16 for FP32, default), "sweep.depth" (1 or 2 planes of
  * loads in flight per wave, kernel 2, default 1), "sweep.pair" (1: the
  * GaussSeidel smoother runs its sweeps in pairs through mgp_rbgs_sweep2_*,
- * default; 0: one pass per sweep), "sweep2.rows" (register rows per two-sweep
+ * default; 0: one pass per sweep), "sweep.pair_min_points" (levels with fewer
+ * points run one pass per sweep; default 2^25), "sweep.small_points" and
+ * "sweep.small_min_chunk" (levels with fewer points than the first use the
+ * second as their shortest x march; defaults 2^18 and 1), "sweep2.rows"
+ * (register rows per two-sweep
  * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16, default),
  * "sweep2.chunks_per_tile" (least x chunks per two-sweep wave tile, made odd,
  * and at least enough for about 6800 waves; default 15), "residual.target_wgs",

@@ -124,6 +124,9 @@ int mgp_tuning_set(const char* key, int64_t value) {
     else if (!strcmp(key, "residual.target_wgs") && value <= (1 << 20)) mgp::g_resid_target = value;
     else if (!strcmp(key, "residual_fw.target_wgs") && value <= (1 << 20)) mgp::g_fw_target = value;
     else if (!strcmp(key, "prolong.target_wgs") && value <= (1 << 20)) mgp::g_prolong_target = value;
+    else if (!strcmp(key, "sweep.small_points")) mgp::g_small_points = value;
+    else if (!strcmp(key, "sweep.small_min_chunk") && value >= 1) mgp::g_small_min_chunk = value;
+    else if (!strcmp(key, "sweep.pair_min_points")) mgp::g_pair_min_points = value;
     else if (!strcmp(key, "sweep.waves") && (value == 4 || value == 8 || value == 16)) mgp::g_sweep_waves = value;
     else if (!strcmp(key, "sweep.kernel") && (value == 1 || value == 2)) mgp::g_sweep_kernel = value;
     else if (!strcmp(key, "sweep.rows") && (value == 0 || value == 8 || value == 12 || value == 16))
@@ -151,6 +154,9 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "residual.target_wgs")) *value = mgp::g_resid_target;
     else if (!strcmp(key, "residual_fw.target_wgs")) *value = mgp::g_fw_target;
     else if (!strcmp(key, "prolong.target_wgs")) *value = mgp::g_prolong_target;
+    else if (!strcmp(key, "sweep.small_points")) *value = mgp::g_small_points;
+    else if (!strcmp(key, "sweep.small_min_chunk")) *value = mgp::g_small_min_chunk;
+    else if (!strcmp(key, "sweep.pair_min_points")) *value = mgp::g_pair_min_points;
     else if (!strcmp(key, "sweep.waves")) *value = mgp::g_sweep_waves;
     else if (!strcmp(key, "sweep.kernel")) *value = mgp::g_sweep_kernel;
     else if (!strcmp(key, "sweep.rows")) *value = mgp::g_sweep_rows;

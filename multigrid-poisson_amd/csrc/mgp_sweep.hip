@@ -50,6 +50,23 @@ int64_t g_prolong_target = 65536;
 extern int64_t g_sweep_target_wgs;
 static inline int64_t target_wgs(int64_t own) { return own > 0 ? own : g_sweep_target_wgs; }
 
+// shortest x march of a launch over a box of `points` points: levels below
+// "sweep.small_points" points are latency-bound marches whose cost grows with
+// the chunk length (65 x 65 x 33 and smaller, FP64: one-pass sweep 31 us with
+// chunks of 4 planes, 17 us with 1), so they use "sweep.small_min_chunk"
+int64_t g_small_points = 1 << 18;
+int64_t g_small_min_chunk = 1;
+// the Gauss-Seidel smoother pairs its sweeps (mgp_rbgs_sweep2_*) on levels of
+// at least this many points; below it two one-pass sweeps are faster (FP64:
+// 257 x 257 x 129 pair 228 us, two sweeps 169 us; 513 x 513 x 257 pair
+// 0.83 ms, two sweeps 0.94 ms)
+int64_t g_pair_min_points = 1 << 25;
+extern int64_t g_sweep_min_chunk;
+static inline int64_t min_chunk_for(int64_t points) {
+    return points < g_small_points ? g_small_min_chunk : g_sweep_min_chunk;
+}
+static inline int64_t box_points(const int64_t b[6]) { return (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]); }
+
 namespace {
 
 constexpr int RW = 64;        // ring-1 rectangle width (z)
@@ -1544,7 +1561,7 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
         A.ztiles = (int)((b[5] - b[4] + WZ - 1) / WZ);
         const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
         int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
-        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
         A.xchunk = (nx + chunks - 1) / chunks;
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
         A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
@@ -1619,7 +1636,7 @@ int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mg
         // fewer, longer marches; a stored residual keeps the sweep target
         const int64_t tw = store ? g_sweep_target_wgs : target_wgs(g_resid_target);
         int64_t chunks = std::max<int64_t>(1, tw / std::max<int64_t>(tiles, 1));
-        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
         A.xchunk = (nx + chunks - 1) / chunks;
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
         A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
@@ -1698,7 +1715,7 @@ int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, con
         A.ztiles = (int)((nz + RZ - 1) / RZ);
         const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
         int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
-        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
         A.xchunk = (nx + chunks - 1) / chunks;
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
         A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
@@ -1767,7 +1784,7 @@ int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stre
     A.ztiles = (int)((A.u.n2 + 63) / 64);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
     int64_t chunks = std::max<int64_t>(1, target_wgs(g_prolong_target) / std::max<int64_t>(tiles, 1));
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 2)));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(A.u.n0 * A.u.n1 * A.u.n2), 2)));
     int64_t xchunk = (nx + chunks - 1) / chunks;
     xchunk += xchunk & 1;  // chunks start on even planes
     A.xchunk = xchunk;
@@ -1856,7 +1873,7 @@ int64_t march_xchunk(const int64_t b[6]) {
     const int64_t tiles = ((b[3] - b[2] + TY - 1) / TY) * ((b[5] - b[4] + TZ - 1) / TZ);
     const int64_t nx = b[1] - b[0];
     int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
     return (nx + chunks - 1) / chunks;
 }
 
@@ -1902,7 +1919,7 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     // and below flat.
     const int64_t tl = std::max<int64_t>(tiles, 1);
     int64_t chunks = std::max<int64_t>(g_sweep2_cpt, (6800 + tl - 1) / tl) | 1;
-    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(g_sweep_min_chunk, 1)));
+    chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
     A.xchunk = (nx + chunks - 1) / chunks;
     A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
     A.ntiles_pad = (A.ntiles + 7) / 8 * 8;

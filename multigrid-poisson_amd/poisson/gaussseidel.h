@@ -7,7 +7,7 @@
 // (mgp_rbgs_sweep) that forms Neumann ghosts on the fly; the values are
 // identical to the two half-sweeps with boundary refreshes in between; pairs
 // of such sweeps run as one pass (mgp_rbgs_sweep2) unless the "sweep.pair"
-// tuning key is 0.
+// tuning key is 0 or the level has fewer points than "sweep.pair_min_points".
 #ifndef POISSON_GAUSS_SEIDEL
 #define POISSON_GAUSS_SEIDEL
 
@@ -60,8 +60,12 @@ public:
             }
         }
         int_t s = 0;
-        int64_t pair = 1;
+        int64_t pair = 1, pair_min = 0;
         if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
+        // pairs pay on large levels only (two one-pass sweeps are faster below)
+        if (mgp_tuning_get("sweep.pair_min_points", &pair_min) == MGP_OK &&
+            d.u->shape[0] * d.u->shape[1] * d.u->shape[2] < pair_min)
+            pair = 0;
         // pairs of sweeps in one pass each (MGP_ERR_ARG: a shape the two-sweep
         // pass does not take; single sweeps then)
         for (; pair && s + 2 <= nsmooth; s += 2) {

@@ -682,23 +682,29 @@ def test_two_sweep_pass_rejects_in_place_and_thin_levels():
 @pytest.mark.parametrize("dtype", [mgp.F64, mgp.MIXED], ids=["f64", "mixed"])
 @pytest.mark.parametrize("nsmooth", [2, 3])
 def test_paired_sweeps_solve_equals_single_sweep_solve(dtype, nsmooth):
-    """A solve whose smoother runs sweep pairs through the two-sweep pass
-    (sweep.pair 1) ends with the same bits as one pass per sweep (sweep.pair 0):
-    iterations, relative residual and solution."""
+    """A solve whose smoother runs sweep pairs through the two-sweep pass on
+    every level (sweep.pair 1, sweep.pair_min_points 0) ends with the same bits
+    as one pass per sweep (sweep.pair 0), and as the default launch shapes
+    (pairs on large levels only, one-plane chunks on small ones) and as
+    4-plane chunks everywhere (sweep.small_points 0): iterations, relative
+    residual and solution."""
     say("case pair solve", dtype, nsmooth)
     dims, levels = (65, 65, 33), 4
     out = {}
-    for pair in (0, 1):
-        with tuned(sweep__pair=pair):
+    configs = {"single": dict(sweep__pair=0), "pairs": dict(sweep__pair=1, sweep__pair_min_points=0),
+               "default": dict(), "no_small_chunks": dict(sweep__small_points=0)}
+    for name, kv in configs.items():
+        with tuned(**kv):
             s = mgp.Solver(dims, levels, 1.0 / 64, (0, 0, 0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
                            mgp.FULL_WEIGHTING, 2, dtype=dtype, maxiter=6, miniter=6, tolerance=0.0)
             s.to_device()
             it, rel, _ = s.solve("Fcycle", nsmooth, 1.0)
             s.to_host()
-            out[pair] = (it, rel, s.array("u").copy())
+            out[name] = (it, rel, s.array("u").copy())
             s.close()
-    assert out[0][:2] == out[1][:2]
-    np.testing.assert_array_equal(out[0][2], out[1][2])
+    for name in configs:
+        assert out[name][:2] == out["single"][:2], name
+        np.testing.assert_array_equal(out[name][2], out["single"][2])
 
 
 PROLONG_SHAPES = [(1025 // 16 + 1, 33, 17), (17, 33, 65), (9, 9, 9), (33, 17, 129), (5, 3, 3)]
