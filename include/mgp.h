@@ -481,7 +481,8 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * "residual_fw.target_wgs", "prolong.target_wgs" (waves the norm-only
  * residual, the fused residual + full-weighting and the prolongation marches
  * aim for; 0 = the value of
- * "sweep.target_wgs"; defaults 4096, 32768, 65536), "residual.kernel" (1:
+ * "sweep.target_wgs"; defaults 4096, 32768, 65536), "update_residual.target_wgs"
+ * (waves of mgp_update_residual_f64_f32; default 65536), "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
  * "residual.rows" (8 or 16 register rows per residual tile, default 16),
  * "residual_fw.rows" (4 or 6 coarse rows per wave tile of the fused residual

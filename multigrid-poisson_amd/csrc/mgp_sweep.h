@@ -52,7 +52,7 @@ int64_t march_xchunk(const int64_t box[6]);
 extern int64_t g_batch_blocks;  // workgroups per batched launch (mgp_batch.hip)
 extern int64_t g_sweep_target_wgs;
 extern int64_t g_sweep2_cpt;
-extern int64_t g_resid_target, g_fw_target, g_prolong_target;
+extern int64_t g_resid_target, g_fw_target, g_prolong_target, g_upd_target;
 extern int64_t g_small_points, g_small_min_chunk, g_pair_min_points;
 extern int64_t g_sweep_min_chunk;
 extern int64_t g_sweep_waves;

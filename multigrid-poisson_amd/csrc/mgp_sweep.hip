@@ -47,6 +47,9 @@ namespace mgp {
 int64_t g_resid_target = 4096;
 int64_t g_fw_target = 32768;
 int64_t g_prolong_target = 65536;
+// the FP64 update + residual of mixed-precision refinement: 4.59 ms with 16384
+// waves, 4.48 ms with 65536 (1025 x 1025 x 513)
+int64_t g_upd_target = 65536;
 extern int64_t g_sweep_target_wgs;
 static inline int64_t target_wgs(int64_t own) { return own > 0 ? own : g_sweep_target_wgs; }
 
@@ -1714,7 +1717,7 @@ int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, con
         A.ytiles = (int)((ny + R - 1) / R);
         A.ztiles = (int)((nz + RZ - 1) / RZ);
         const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
-        int64_t chunks = std::max<int64_t>(1, g_sweep_target_wgs / std::max<int64_t>(tiles, 1));
+        int64_t chunks = std::max<int64_t>(1, target_wgs(g_upd_target) / std::max<int64_t>(tiles, 1));
         chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
         A.xchunk = (nx + chunks - 1) / chunks;
         A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
