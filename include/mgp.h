@@ -114,6 +114,29 @@ int mgp_update_residual_f64_f32(int mode, const mgp_array* u, const mgp_array* u
                                 const mgp_array* f, const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
                                 const int64_t box[6], double* norm, void* stream);
 
+/* The iterate of refinement held as hi + lo: FP64 hi plus an FP32 lo word
+ * (same shape, halos and element strides as hi), about 77 significant bits,
+ * so that the FP64 residual is not floored by the rounding of u to FP64
+ * (12/h^2 ulp(u), about 1e-10 of ||f|| at h = 1/1024).  Consistent mode.
+ * Update and residual in one pass: (u_new, lo_new) = u + lo + (double)e
+ * renormalised by an error-free sum (hi' = fl(u + fl(e + lo)), lo' = the
+ * rounding error as FP32), then out = (float)(f - L(u_new + lo_new)) on the
+ * box, S(hi) + S(lo) summed in FP64 (S = the consistent-mode sum of
+ * neighbour differences), Neumann ghosts formed from `faces` with the
+ * rounding of sign*2h*g + hi(mirror) carried into lo; with norm, max |r|
+ * over the box and r's stored values outside it.  u_new and lo_new are
+ * second buffers (halos not written).  40 bytes per point.  No counterpart
+ * in the reference (FP64 only there; its residual is include/residual.h:58-79). */
+int mgp_update_residual_hilo_f64_f32(const mgp_array* u, const mgp_array* u_new, const mgp_array* lo,
+                                     const mgp_array* lo_new, const mgp_array* e, const mgp_array* f,
+                                     const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
+                                     const int64_t box[6], double* norm, void* stream);
+/* r = f - L(u + lo) on the box written in FP64 (the residual of the stored
+ * hi + lo iterate, as the stop rule saw it), ghosts from `faces`, with norm
+ * as above */
+int mgp_residual_hilo_f64(const mgp_array* u, const mgp_array* lo, const mgp_array* f, const mgp_array* r,
+                          const mgp_faces* faces, const int64_t box[6], double* norm, void* stream);
+
 /* ------------------------------------------- FP16 storage of coarse levels
  * (no counterpart in the reference: its T is one type throughout).  Arrays
  * hold IEEE binary16; values are loaded as float, combined with the
@@ -165,7 +188,8 @@ int mgp_rbgs_half_f32(int mode, const mgp_array* u, const mgp_array* f, const in
  * leave in the shape; the halo of u_out is not written (ghosts are formed
  * again before every use).  Requires a
  * checkerboard colouring: always in consistent mode; in reference mode only
- * when stride[0] and stride[1] are odd (MGP_ERR_ARG otherwise). */
+ * when stride[0] and stride[1] are odd (MGP_ERR_ARG otherwise).  An empty box
+ * is a no-op in place and MGP_ERR_ARG out of place (nothing would be copied). */
 int mgp_rbgs_sweep_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                        const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 int mgp_rbgs_sweep_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
@@ -176,7 +200,8 @@ int mgp_rbgs_sweep_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, 
  * single counterpart in the reference), shape points outside the box copied
  * as above.  u_out must be a different buffer; MGP_ERR_ARG also where the
  * level has fewer than
- * 3 x-planes or a plane of 1 GiB or more (run single sweeps there). */
+ * 3 x-planes or a plane of 1 GiB or more (run single sweeps there), and for an
+ * empty box. */
 int mgp_rbgs_sweep2_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                         const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 int mgp_rbgs_sweep2_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
@@ -185,11 +210,19 @@ int mgp_rbgs_sweep2_f32(int mode, const mgp_array* u_in, const mgp_array* u_out,
  * workgroup's LDS ((nx+2)(ny+2)(nz+2) + nx ny nz elements <= 64 KiB;
  * MGP_ERR_ARG otherwise): the result of the last sweep goes to u_out, which may
  * be u_in.  Same values as nsweeps calls of mgp_rbgs_sweep_* (the coarsest
- * level's smoothing, where launch latency dominates). */
+ * level's smoothing, where launch latency dominates).  An empty box or
+ * nsweeps = 0 is a no-op in place and MGP_ERR_ARG out of place. */
 int mgp_rbgs_sweeps_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                         const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
 int mgp_rbgs_sweeps_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                         const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
+/* The same on FP32 arrays with FP64 arithmetic: u and f widened (exactly) to
+ * double in LDS, the nsweeps sweeps in FP64, the result rounded once to FP32
+ * (the coarsest-level solve of FP32 cycles inside FP64 refinement; no
+ * counterpart in the reference).  The level must fit the LDS limit above with
+ * 8-byte elements. */
+int mgp_rbgs_sweeps_f32_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                            const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
 /* Weighted Jacobi over the box: u = (1-w)v + w/6 (sum_nb v - h^2 f)
  * (include/jacobi.h:39-82).  v's halos must hold current ghost values. */
 int mgp_jacobi_f64(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t box[6],
@@ -265,6 +298,12 @@ enum { MGP_JACOBI = 0, MGP_GAUSS_SEIDEL = 1 };
 #define MGP_SOLVER_NO_FUSED 2  /* red-black half-sweeps only (Settings::fused_sweeps = false) */
 #define MGP_SOLVER_QUIET 4     /* no per-cycle timing line, no level listing at creation
                                   (Settings::print_cycle_time, print_settings = false) */
+/* MGP_MIXED: the iterate as one FP64 array (no FP32 lo word; the FP64
+ * residual then floors near 12/h^2 ulp(u)) (Settings::refine_hilo = false) */
+#define MGP_SOLVER_PLAIN_ITERATE 8
+/* MGP_MIXED: the FP32 cycle's coarsest level smoothed in FP32 arithmetic
+ * instead of FP64 (Settings::refine_coarse_fp64 = false) */
+#define MGP_SOLVER_COARSE_FP32 16
 
 typedef struct mgp_solver_config {
     int64_t dims[3];     /* Settings::dims (include/settings.h:17) */
@@ -319,7 +358,9 @@ int mgp_solver_reset(void* solver);
 /* relative_residual(), solve_time(), solve_iterations() (libpoisson.h:378-393) */
 int mgp_solver_stats(void* solver, double* rel_res, double* seconds, int64_t* iterations);
 /* host mirror and device descriptor of level `level`'s u ('u'), uprev ('p'),
- * f ('f') or r ('r') */
+ * f ('f') or r ('r'); for MGP_MIXED also 'l': the FP32 lo word of the hi + lo
+ * iterate on level 0 (device only, *host = NULL; MGP_ERR_ARG with
+ * MGP_SOLVER_PLAIN_ITERATE) */
 int mgp_solver_array(void* solver, int level, int which, void** host, mgp_array* device);
 /* host mirror and device descriptor of a face array; *type receives the
  * effective boundary type (MGP_DIRICHLET / MGP_NEUMANN) */
@@ -482,7 +523,9 @@ int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int
  * residual, the fused residual + full-weighting and the prolongation marches
  * aim for; 0 = the value of
  * "sweep.target_wgs"; defaults 4096, 32768, 65536), "update_residual.target_wgs"
- * (waves of mgp_update_residual_f64_f32; default 65536), "residual.kernel" (1:
+ * (waves of mgp_update_residual_f64_f32 and the hi + lo kernels; default 65536),
+ * "update_residual.rows" (8 or 16 register rows per wave tile of the hi + lo
+ * kernels, default 16), "residual.kernel" (1:
  * per-point kernels; 2: one wave per tile marching along x, default),
  * "residual.rows" (8 or 16 register rows per residual tile, default 16),
  * "residual_fw.rows" (4 or 6 coarse rows per wave tile of the fused residual

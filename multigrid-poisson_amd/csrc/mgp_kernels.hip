@@ -199,10 +199,12 @@ __global__ void k_gs_half(View<T> u, View<T> f, Box bx, int64_t zc, T one_omega,
 // half-sweep (include/relaxation.h:25-90), with barriers between, exactly
 // what the per-sweep kernels compute; the box points are written to uo at
 // the end.  Latency, not bandwidth, bounds the coarsest levels, and one
-// launch replaces 2 nsweeps of them.
-template <class T, int MODE>
-__global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<T> f, Faces<T> F, Box bx,
-                                                    int64_t nsweeps, T one_omega, T omega_sixth, T hsq,
+// launch replaces 2 nsweeps of them.  S is the arrays' storage type, T the
+// arithmetic (and LDS) type: S = float with T = double is the FP64 solve of
+// an FP32 coarsest level (values widen exactly on load, round once on store).
+template <class S, class T, int MODE>
+__global__ void __launch_bounds__(1024) k_rbgs_lds(View<S> ui, View<S> uo, View<S> f, Faces<S> F, Box bx,
+                                                    int64_t nsweeps, T one_omega, T omega_sixth, T hsq, T two_h,
                                                     int64_t lin0) {
     constexpr int MAXP = 8;  // box points per thread: <= 8192 / 1024 for any level that fits
     extern __shared__ unsigned char lds_raw[];
@@ -222,8 +224,8 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
     __syncthreads();
     for (int t = tid; t < n0 * n1 * n2; t += nt) {
         const int i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
-        su[L(i, j, k)] = ui.at(i, j, k);
-        sf[t] = f.at(i, j, k);
+        su[L(i, j, k)] = (T)ui.at(i, j, k);
+        sf[t] = (T)f.at(i, j, k);
     }
     // this thread's box points: LDS index, f index, colour
     const int bn0 = (int)(bx.x1 - bx.x0), bn1 = (int)(bx.y1 - bx.y0), bn2 = (int)(bx.z1 - bx.z0);
@@ -273,7 +275,7 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
         }
         gc[q] = L(gi, gj, gk);
         gd[q] = d;
-        gv[q] = sign * F.two_h * F.g[q][t];
+        gv[q] = sign * two_h * (T)F.g[q][t];
     }
     __syncthreads();
     for (int64_t s = 0; s < nsweeps; s++) {
@@ -297,7 +299,7 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
         if (pc[m] < 0) continue;
         const int t = tid + m * nt;
         const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
-        uo.at(i, j, k) = su[pc[m]];
+        uo.at(i, j, k) = (S)su[pc[m]];
     }
     // out of place: the shape points outside the box as loaded, so u_out is a
     // complete image of the in-place result (the boundary points' own history,
@@ -306,7 +308,7 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<T> ui, View<T> uo, View<
         for (int t = tid; t < n0 * n1 * n2; t += nt) {
             const int i = t / (n1 * n2), j = (t / n2) % n1, k = t % n2;
             if (i >= bx.x0 && i < bx.x1 && j >= bx.y0 && j < bx.y1 && k >= bx.z0 && k < bx.z1) continue;
-            uo.at(i, j, k) = su[L(i, j, k)];
+            uo.at(i, j, k) = (S)su[L(i, j, k)];
         }
     }
 }
@@ -766,7 +768,8 @@ static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const 
         (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
         (bx.z1 == ui->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
         return MGP_ERR_ARG;
-    if (box_empty(bx)) return MGP_OK;
+    // an empty box out of place would leave u_out without the copy of u_in
+    if (box_empty(bx)) return ui->data == uo->data ? MGP_OK : MGP_ERR_ARG;
     return march_sweep<T>(mode, ui, uo, f, faces, b, omega, march_xchunk(b), stream);
 }
 
@@ -792,7 +795,7 @@ static int rbgs_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const
         return MGP_ERR_ARG;
     for (int q = 0; q < 6; q++)
         if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
-    if (box_empty(bx)) return MGP_OK;
+    if (box_empty(bx)) return MGP_ERR_ARG;  // u_out would not receive the copy of u_in
     return march_sweep2<T>(mode, ui, uo, f, faces, b, omega, stream);
 }
 
@@ -803,7 +806,7 @@ static inline size_t lds_sweeps_bytes(const mgp_array* u, size_t esize) {
     return b <= 65536 ? b : 0;
 }
 
-template <class T>
+template <class S, class T = S>
 static int rbgs_sweeps(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
                        const int64_t b[6], double omega, int64_t nsweeps, void* stream) {
     if (!valid_array(ui) || !valid_array(uo) || !valid_array(f) || !faces || !same_shape(ui, uo) ||
@@ -829,20 +832,18 @@ static int rbgs_sweeps(int mode, const mgp_array* ui, const mgp_array* uo, const
     if ((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) > 8 * 1024) return MGP_ERR_ARG;  // 8 points per thread
     const int64_t n0 = ui->shape[0], n1 = ui->shape[1], n2 = ui->shape[2];
     if (n1 * n2 > 1024 || n0 * n2 > 1024 || n0 * n1 > 1024) return MGP_ERR_ARG;  // one ghost per thread and face
-    if (box_empty(bx) || nsweeps == 0) {
-        if (ui->data != uo->data && !box_empty(bx)) return MGP_ERR_ARG;
-        return MGP_OK;
-    }
+    if (box_empty(bx) || nsweeps == 0) return ui->data == uo->data ? MGP_OK : MGP_ERR_ARG;
     const RelaxConsts<T> c(omega, faces->h);
-    const Faces<T> F = faces_of<T>(*faces);
-    const View<T> vi = view_of<T>(*ui);
+    const Faces<S> F = faces_of<S>(*faces);
+    const T two_h = (T)(2.0 * faces->h);  // include/neumann.h:108, in the arithmetic type
+    const View<S> vi = view_of<S>(*ui);
     const hipStream_t s = (hipStream_t)stream;
     if (mode == MGP_MODE_REFERENCE)
-        hipLaunchKernelGGL((k_rbgs_lds<T, MGP_MODE_REFERENCE>), dim3(1), dim3(1024), lds, s, vi, view_of<T>(*uo),
-                           view_of<T>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, lin0_of(ui));
+        hipLaunchKernelGGL((k_rbgs_lds<S, T, MGP_MODE_REFERENCE>), dim3(1), dim3(1024), lds, s, vi, view_of<S>(*uo),
+                           view_of<S>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, two_h, lin0_of(ui));
     else
-        hipLaunchKernelGGL((k_rbgs_lds<T, MGP_MODE_CONSISTENT>), dim3(1), dim3(1024), lds, s, vi, view_of<T>(*uo),
-                           view_of<T>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, lin0_of(ui));
+        hipLaunchKernelGGL((k_rbgs_lds<S, T, MGP_MODE_CONSISTENT>), dim3(1), dim3(1024), lds, s, vi, view_of<S>(*uo),
+                           view_of<S>(*f), F, bx, nsweeps, c.one_omega, c.omega_sixth, c.hsq, two_h, lin0_of(ui));
     return launch_status();
 }
 
@@ -1160,6 +1161,43 @@ int mgp_update_residual_f64_f32(int m, const mgp_array* u, const mgp_array* u_ne
     return march_update_residual(m, u, u_new, e, f, r, out, faces, b, norm, s);
 }
 
+// box and faces of the hi + lo kernels: a box point on a face needs that
+// face's ghost (Neumann with its data)
+static bool hilo_faces_ok(const mgp_array* u, const mgp_faces* faces, const int64_t b[6]) {
+    if (!faces || !(faces->h > 0.0) || !valid_box(u, b)) return false;
+    const Box bx = box_of(b);
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == u->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == u->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == u->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return false;
+    for (int q = 0; q < 6; q++)
+        if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return false;
+    return true;
+}
+
+int mgp_update_residual_hilo_f64_f32(const mgp_array* u, const mgp_array* u_new, const mgp_array* lo,
+                                     const mgp_array* lo_new, const mgp_array* e, const mgp_array* f,
+                                     const mgp_array* r, const mgp_array* out, const mgp_faces* faces,
+                                     const int64_t b[6], double* norm, void* s) {
+    const mgp_array* all[] = {u, u_new, lo, lo_new, e, f, r, out};
+    for (const mgp_array* a : all)
+        if (!valid_array(a) || !same_shape(u, a)) return MGP_ERR_ARG;
+    if (u->data == u_new->data || lo->data == lo_new->data || !hilo_faces_ok(u, faces, b)) return MGP_ERR_ARG;
+    return march_hilo_residual(u, u_new, lo, lo_new, e, f, r, out, faces, b, norm, s);
+}
+
+int mgp_residual_hilo_f64(const mgp_array* u, const mgp_array* lo, const mgp_array* f, const mgp_array* r,
+                          const mgp_faces* faces, const int64_t b[6], double* norm, void* s) {
+    const mgp_array* all[] = {u, lo, f, r};
+    for (const mgp_array* a : all)
+        if (!valid_array(a) || !same_shape(u, a)) return MGP_ERR_ARG;
+    if (r->data == u->data || !hilo_faces_ok(u, faces, b)) return MGP_ERR_ARG;
+    return march_hilo_residual(u, nullptr, lo, nullptr, nullptr, f, r, r, faces, b, norm, s);
+}
+
 int mgp_add_f32_to_f64(const mgp_array* u, const mgp_array* e, void* s) {
     if (!valid_array(u) || !valid_array(e) || !same_shape(u, e)) return MGP_ERR_ARG;
     const Box bx = shape_box(u);
@@ -1266,6 +1304,10 @@ int mgp_rbgs_sweeps_f64(int m, const mgp_array* ui, const mgp_array* uo, const m
 int mgp_rbgs_sweeps_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                         const int64_t b[6], double w, int64_t n, void* s) {
     return rbgs_sweeps<float>(m, ui, uo, f, F, b, w, n, s);
+}
+int mgp_rbgs_sweeps_f32_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                            const int64_t b[6], double w, int64_t n, void* s) {
+    return rbgs_sweeps<float, double>(m, ui, uo, f, F, b, w, n, s);
 }
 int mgp_jacobi_f64(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], double w,
                    double h, void* s) {

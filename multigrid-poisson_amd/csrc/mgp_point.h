@@ -55,6 +55,49 @@ __device__ __forceinline__ T residual_point(const View<T>& u, int64_t c, T fc, T
     }
 }
 
+// ---- iterate held as hi + lo (FP64 hi, FP32 lo) in mixed-precision
+// refinement: a solution carried to about 2^-77 relative, so that the FP64
+// residual of it is not limited by the rounding of u to FP64 (no counterpart
+// in the reference, whose u is one array of T).
+
+// Error-free sum (Knuth's TwoSum): s = fl(a + b) and a + b = s + e exactly
+// (needs separate roundings: the including files build with -ffp-contract=off)
+__device__ __forceinline__ double two_sum(double a, double b, double& e) {
+    const double s = a + b;
+    const double bb = s - a;
+    e = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+// The refinement update u += e of u = hi + lo: hi' = fl(hi + t) with
+// t = fl(e + lo), lo' = (float) of the exact rounding error of hi'
+__device__ __forceinline__ void hilo_update(double& hi, float& lo, float e) {
+    double err;
+    hi = two_sum(hi, (double)e + (double)lo, err);
+    lo = (float)err;
+}
+
+// Residual of u = hi + lo at one point, consistent-mode order:
+// r = fma(-1/h^2, S(hi) + S(lo), f), S(v) = ((vW-vc)+(vE-vc)) + ((vS-vc)+(vN-vc))
+// + ((vB-vc)+(vT-vc)); the lo differences are taken in FP64
+__device__ __forceinline__ double hilo_residual_point(double hc, double hW, double hE, double hS, double hN, double hB,
+                                                      double hT, float lc, float lW, float lE, float lS, float lN,
+                                                      float lB, float lT, double fc, double inv_h2) {
+    const double sh = ((hW - hc) + (hE - hc)) + ((hS - hc) + (hN - hc)) + ((hB - hc) + (hT - hc));
+    const double c = (double)lc;
+    const double sl = (((double)lW - c) + ((double)lE - c)) + (((double)lS - c) + ((double)lN - c)) +
+                      (((double)lB - c) + ((double)lT - c));
+    return fma(-inv_h2, sh + sl, fc);
+}
+
+// Neumann ghost of hi + lo: (p + mirror) with p = sign*2h*g, the rounding of
+// p + hi(mirror) carried into lo
+__device__ __forceinline__ void hilo_ghost(double p, double hm, float lm, double& hg, float& lg) {
+    double err;
+    hg = two_sum(p, hm, err);
+    lg = (float)((double)lm + err);
+}
+
 // The grid transfers below take any view type V with at(i,j,k) and n0..n2
 // (View<T>, or FP16 storage read as float).
 //

@@ -125,6 +125,7 @@ int mgp_tuning_set(const char* key, int64_t value) {
     else if (!strcmp(key, "residual_fw.target_wgs") && value <= (1 << 20)) mgp::g_fw_target = value;
     else if (!strcmp(key, "prolong.target_wgs") && value <= (1 << 20)) mgp::g_prolong_target = value;
     else if (!strcmp(key, "update_residual.target_wgs") && value <= (1 << 20)) mgp::g_upd_target = value;
+    else if (!strcmp(key, "update_residual.rows") && (value == 8 || value == 16)) mgp::g_hilo_rows = value;
     else if (!strcmp(key, "sweep.small_points")) mgp::g_small_points = value;
     else if (!strcmp(key, "sweep.small_min_chunk") && value >= 1) mgp::g_small_min_chunk = value;
     else if (!strcmp(key, "sweep.pair_min_points")) mgp::g_pair_min_points = value;
@@ -156,6 +157,7 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "residual_fw.target_wgs")) *value = mgp::g_fw_target;
     else if (!strcmp(key, "prolong.target_wgs")) *value = mgp::g_prolong_target;
     else if (!strcmp(key, "update_residual.target_wgs")) *value = mgp::g_upd_target;
+    else if (!strcmp(key, "update_residual.rows")) *value = mgp::g_hilo_rows;
     else if (!strcmp(key, "sweep.small_points")) *value = mgp::g_small_points;
     else if (!strcmp(key, "sweep.small_min_chunk")) *value = mgp::g_small_min_chunk;
     else if (!strcmp(key, "sweep.pair_min_points")) *value = mgp::g_pair_min_points;

@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -142,8 +143,12 @@ class Server {
 public:
     explicit Server(const mgp_server_config& c) : cfg(c) {
         if (c.hbm_budget <= 0 || c.max_active < 1 || c.nsmooth < 1 || c.maxiter < 1 || !(c.tolerance >= 0.0) ||
-            (c.relaxation != MGP_GAUSS_SEIDEL && c.relaxation != MGP_JACOBI))
+            (c.relaxation != MGP_GAUSS_SEIDEL && c.relaxation != MGP_JACOBI) || !std::isfinite(c.omega))
             throw std::invalid_argument("invalid server config");
+        // omega <= 0 stands for the smoother's default (include/jacobi.h:29-31,
+        // include/gaussseidel.h:97-99 of the reference); the solver itself only
+        // treats negative values so, and omega = 0 would smooth nothing
+        if (cfg.omega <= 0.0) cfg.omega = c.relaxation == MGP_JACOBI ? 6.0 / 7.0 : 8.0 / 9.0;
         int dev = 0;
         hip_check(hipGetDevice(&dev), "server");
         for (int w = 0; w < c.max_active; w++) workers.emplace_back([this, dev] { run(dev); });
@@ -284,8 +289,8 @@ private:
                 if (!s) {
                     s = make_slot(j->req);
                     std::lock_guard<std::mutex> lk(mu);
-                    // replace the estimate by what the allocation took
-                    bytes_used = bytes_used - reserved + s->bytes;
+                    // the reservation now belongs to the slot (s->bytes is the
+                    // same estimate, need_of) and leaves with it
                     reserved = 0;
                 }
                 s->begin(j->req);
@@ -299,12 +304,21 @@ private:
                 mgp::set_last_error(e.what());
                 status = MGP_ERR_STATE;
             }
+            // a slot whose request failed may hold a broken stream or graph:
+            // it is destroyed, not kept for later requests
+            std::unique_ptr<SlotBase> broken;
             {
                 std::lock_guard<std::mutex> lk(mu);
                 bytes_used -= reserved;
-                if (s) idle.push_back(std::move(s));
+                if (s && status == MGP_OK) {
+                    idle.push_back(std::move(s));
+                } else if (s) {
+                    bytes_used -= s->bytes;
+                    broken = std::move(s);
+                }
                 active--;
             }
+            broken.reset();
             cv_work.notify_all();  // memory or a worker came free
             finish_job(j, status);
         }

@@ -114,6 +114,13 @@ struct MixedImpl : SolverBase {
         if (level != 0) return MGP_ERR_ARG;  // the FP64 problem lives on the finest level only
         Poisson::Domain<double>& d = s.fp64().get_domain(0, 0);
         Poisson::DeviceArray<double>* a = nullptr;
+        if (which == 'l') {  // FP32 lo word of the hi + lo iterate (device only)
+            const mgp_array lo = s.solution_lo();
+            if (!lo.data) return MGP_ERR_ARG;
+            if (host) *host = nullptr;
+            if (dev) *dev = lo;
+            return MGP_OK;
+        }
         switch (which) {
         case 'u': a = d.u; break;
         case 'p': a = d.uprev; break;
@@ -201,6 +208,8 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
         st.fused_sweeps = !(c->flags & MGP_SOLVER_NO_FUSED);
         st.print_cycle_time = !(c->flags & MGP_SOLVER_QUIET);
         st.print_settings = !(c->flags & MGP_SOLVER_QUIET);
+        st.refine_hilo = !(c->flags & MGP_SOLVER_PLAIN_ITERATE);
+        st.refine_coarse_fp64 = !(c->flags & MGP_SOLVER_COARSE_FP32);
         if (c->refine_polish < 0) throw std::invalid_argument("refine_polish must be >= 0");
         st.refine_polish = c->refine_polish;
         if (c->half_from < 0) throw std::invalid_argument("half_from must be >= 0");

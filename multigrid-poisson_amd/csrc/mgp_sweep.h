@@ -34,6 +34,14 @@ int march_residual_to(int mode, const mgp_array* u, const mgp_array* f, const mg
 int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, const mgp_array* e, const mgp_array* f,
                           const mgp_array* r, const mgp_array* out, const mgp_faces* faces, const int64_t box[6],
                           double* norm, void* stream);
+// iterate held as hi + lo (consistent mode): with e, (un, ln) = hi + lo + e
+// renormalised and (float)(f - L(un + ln)) into out; without e (e, un, ln
+// NULL), (double)(f - L(u + lo)) into out.  Ghosts formed from faces, max |r|
+// into *norm (r's stored values outside the box included)
+int march_hilo_residual(const mgp_array* u, const mgp_array* un, const mgp_array* lo, const mgp_array* ln,
+                        const mgp_array* e, const mgp_array* f, const mgp_array* r, const mgp_array* out,
+                        const mgp_faces* faces, const int64_t box[6], double* norm, void* stream);
+extern int64_t g_hilo_rows;
 // u += trilinear P(coarse) over the fine shape through wave tiles
 template <class T>
 int march_prolong_add(const mgp_array* coarse, const mgp_array* fine, void* stream);

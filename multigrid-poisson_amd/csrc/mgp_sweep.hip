@@ -32,6 +32,7 @@
 
 #include "mgp.h"
 #include "mgp_device.h"
+#include "mgp_point.h"
 #include "mgp_sweep.h"
 
 namespace mgp {
@@ -1214,6 +1215,241 @@ __global__ void __launch_bounds__(64) k_update_residual_wave(UpdArgs A) {
     }
 }
 
+// The iterate of mixed-precision refinement held as hi + lo (FP64 hi, FP32
+// lo; mgp_update_residual_hilo_f64_f32, mgp_residual_hilo_f64).  UPD: the
+// update hi + lo + e, renormalised by an error-free sum, goes to a second pair
+// of buffers (hn, ln), then r = f - L(hi + lo) of the updated iterate is
+// written as FP32 into `out` (the next right-hand side); !UPD: r of the
+// stored iterate written as FP64 into `out`.  The march of
+// k_update_residual_wave; Neumann ghosts of hi + lo formed in registers with
+// the rounding of p + hi(mirror) carried into lo.  Consistent mode only.
+struct HiloArgs {
+    View<double> u, un, f, r;
+    View<float> lo, ln, e;
+    double* out64;
+    float* out32;
+    const double* g[6];
+    double two_h;
+    int64_t bx0, bx1, by0, by1, bz0, bz1;
+    int64_t ylo, yhi, zlo, zhi, xlo, xhi;
+    double inv_h2;
+    unsigned long long* norm;
+    int ytiles, ztiles, ntiles, ntiles_pad;
+    int64_t xchunk;
+};
+
+// shape points outside the box: (hn, ln) = hi + lo + e (the order of
+// copy_complement)
+__device__ void hilo_complement(const HiloArgs& A, int blk) {
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2;
+    const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
+    const int64_t xo = A.bx0 + n0 - A.bx1, yo = A.by0 + n1 - A.by1, zo = A.bz0 + n2 - A.bz1;
+    const int64_t c1 = xo * n1 * n2, c2 = BX * yo * n2, c3 = BX * BY * zo;
+    const int64_t per = (int64_t)blockDim.x * COPY_PER_THREAD;
+    const int64_t end = min(c1 + c2 + c3, (int64_t)(blk + 1) * per);
+    for (int64_t t = (int64_t)blk * per + threadIdx.x; t < end; t += blockDim.x) {
+        int64_t x, y, z;
+        if (t < c1) {
+            const int64_t a = t / (n1 * n2), r = t % (n1 * n2);
+            x = a < A.bx0 ? a : A.bx1 + (a - A.bx0);
+            y = r / n2;
+            z = r % n2;
+        } else if (t < c1 + c2) {
+            const int64_t u = t - c1, a = u / (yo * n2), r = u % (yo * n2), c = r / n2;
+            x = A.bx0 + a;
+            y = c < A.by0 ? c : A.by1 + (c - A.by0);
+            z = r % n2;
+        } else {
+            const int64_t u = t - c1 - c2, a = u / (BY * zo), r = u % (BY * zo), c = r % zo;
+            x = A.bx0 + a;
+            y = A.by0 + r / zo;
+            z = c < A.bz0 ? c : A.bz1 + (c - A.bz0);
+        }
+        double hi = A.u.at(x, y, z);
+        float lo = A.lo.at(x, y, z);
+        hilo_update(hi, lo, A.e.at(x, y, z));
+        A.un.at(x, y, z) = hi;
+        A.ln.at(x, y, z) = lo;
+    }
+}
+
+template <int R, bool UPD>
+__global__ void __launch_bounds__(64) k_hilo_residual_wave(HiloArgs A) {
+    constexpr int NR = R + 2;
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        if (UPD) hilo_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    const int tz = tl % A.ztiles;
+    const int ty = (tl / A.ztiles) % A.ytiles;
+    const int xc = tl / (A.ztiles * A.ytiles);
+    const int64_t y0 = A.by0 + (int64_t)ty * R;
+    const int64_t z0 = A.bz0 + (int64_t)tz * RZ;
+    const int64_t xa = A.bx0 + (int64_t)xc * A.xchunk;
+    const int64_t xb = min(A.bx1, xa + A.xchunk);
+    const int64_t n0 = A.u.n0, n1 = A.u.n1, n2 = A.u.n2, s0 = A.u.s0, s1 = A.u.s1;
+    const double two_h = A.two_h;
+    const uint32_t plane_bytes = (uint32_t)(((A.yhi - A.ylo) * s1 + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(double));
+    const uint32_t fplane_bytes = plane_bytes / 2;
+    const int lane = threadIdx.x;
+    const int64_t z = z0 - 1 + lane;
+    const bool lane_out = lane >= 1 && lane <= RZ && z < A.bz1;
+    const bool z_shape = z >= 0 && z < n2;
+    const int64_t zc = min(max(z, A.zlo), A.zhi);
+    const int64_t zs = min(max(z, (int64_t)0), n2 - 1);  // clamped into the shape (face index)
+    const uint32_t zoff = (uint32_t)((zc - A.zlo) * (int64_t)sizeof(double));
+    uint32_t roff[NR];
+    bool yout[NR];
+#pragma unroll
+    for (int t = 0; t < NR; t++) {
+        const int64_t y = y0 - 1 + t;
+        roff[t] = (uint32_t)((min(max(y, A.ylo), A.yhi) - A.ylo) * s1 * (int64_t)sizeof(double));
+        yout[t] = t >= 1 && t <= R && y < A.by1;
+    }
+    const double* gW = A.g[MGP_WEST];
+    const double* gE = A.g[MGP_EAST];
+    const double* gS = A.g[MGP_SOUTH];
+    const double* gN = A.g[MGP_NORTH];
+    const double* gB = A.g[MGP_BOTTOM];
+    const double* gT = A.g[MGP_TOP];
+    const bool y_ghosts = (y0 - 1 < 0 && gS) || (y0 + R >= n1 && gN);
+    const bool z_ghosts = (z0 - 1 < 0 && gB) || (z0 + RZ >= n2 && gT);
+    const int64_t corner = A.ylo * s1 + A.zlo;
+    // hi + lo (+ e) of plane i, clamped into the allocated range
+    auto load_x = [&](int64_t i, double (&H)[NR], float (&L)[NR]) {
+        const int64_t ic = min(max(i, A.xlo), A.xhi);
+        const __amdgpu_buffer_rsrc_t rh = plane_rsrc(A.u.p + ic * s0 + corner, plane_bytes);
+        const __amdgpu_buffer_rsrc_t rl = plane_rsrc(A.lo.p + ic * s0 + corner, fplane_bytes);
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            H[t] = bload<double>(rh, zoff, roff[t]);
+            L[t] = bload<float>(rl, zoff / 2, roff[t] / 2);
+        }
+        if (UPD) {
+            const __amdgpu_buffer_rsrc_t re = plane_rsrc(A.e.p + ic * s0 + corner, fplane_bytes);
+#pragma unroll
+            for (int t = 0; t < NR; t++) hilo_update(H[t], L[t], bload<float>(re, zoff / 2, roff[t] / 2));
+        }
+    };
+    auto ghosts = [&](int64_t i, double (&H)[NR], float (&L)[NR]) {
+        if (i < 0 || i >= n0) return;  // x-ghost planes are formed whole below
+        if (y_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 1 + t;
+                double hg;
+                float lg;
+                if (t + 2 < NR && y == -1 && gS) {
+                    hilo_ghost((double)-1.0 * two_h * gS[i * n2 + zs], H[t + 2], L[t + 2], hg, lg);  // (nx, 1, nz)
+                    H[t] = z_shape ? hg : H[t];
+                    L[t] = z_shape ? lg : L[t];
+                } else if (t >= 2 && y == n1 && gN) {
+                    hilo_ghost((double)1.0 * two_h * gN[i * n2 + zs], H[t - 2], L[t - 2], hg, lg);
+                    H[t] = z_shape ? hg : H[t];
+                    L[t] = z_shape ? lg : L[t];
+                }
+            }
+        }
+        if (z_ghosts) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t y = y0 - 1 + t;
+                if (y < 0 || y >= n1) continue;
+                double hg;
+                float lg;
+                if (gB) {
+                    hilo_ghost((double)-1.0 * two_h * gB[i * n1 + y], lane_next(lane_next(H[t])),
+                               lane_next(lane_next(L[t])), hg, lg);  // face (nx, ny, 1)
+                    H[t] = z == -1 ? hg : H[t];
+                    L[t] = z == -1 ? lg : L[t];
+                }
+                if (gT) {
+                    hilo_ghost((double)1.0 * two_h * gT[i * n1 + y], lane_prev(lane_prev(H[t])),
+                               lane_prev(lane_prev(L[t])), hg, lg);
+                    H[t] = z == n2 ? hg : H[t];
+                    L[t] = z == n2 ? lg : L[t];
+                }
+            }
+        }
+    };
+    auto load_f = [&](int64_t i, double (&F)[NR]) {
+        const __amdgpu_buffer_rsrc_t rs = plane_rsrc(A.f.p + min(max(i, A.xlo), A.xhi) * s0 + corner, plane_bytes);
+#pragma unroll
+        for (int t = 1; t <= R; t++) F[t] = bload<double>(rs, zoff, roff[t]);
+    };
+    double Hm1[NR], H0[NR], Hp1[NR], HL[NR], F0[NR], FL[NR];
+    float Lm1[NR], L0[NR], Lp1[NR], LL[NR];
+    load_x(xa - 1, Hm1, Lm1);
+    load_x(xa, H0, L0);
+    load_x(xa + 1, Hp1, Lp1);
+    load_f(xa, F0);
+    ghosts(xa - 1, Hm1, Lm1);
+    ghosts(xa, H0, L0);
+    double vmax = 0.0;
+    for (int64_t i = xa; i < xb; i++) {
+        load_x(i + 2, HL, LL);
+        load_f(i + 1, FL);
+        ghosts(i + 1, Hp1, Lp1);
+        // x-ghost planes from the plane two apart (face layout (1, ny, nz))
+        if (i == 0 && gW) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t yc = min(max(y0 - 1 + t, (int64_t)0), n1 - 1);
+                hilo_ghost((double)-1.0 * two_h * gW[yc * n2 + zs], Hp1[t], Lp1[t], Hm1[t], Lm1[t]);
+            }
+        }
+        if (i == n0 - 1 && gE) {
+#pragma unroll
+            for (int t = 0; t < NR; t++) {
+                const int64_t yc = min(max(y0 - 1 + t, (int64_t)0), n1 - 1);
+                hilo_ghost((double)1.0 * two_h * gE[yc * n2 + zs], Hm1[t], Lm1[t], Hp1[t], Lp1[t]);
+            }
+        }
+        const int64_t ic = min(max(i, A.xlo), A.xhi);
+        __amdgpu_buffer_rsrc_t rn, rln, ro;
+        if (UPD) {
+            rn = plane_rsrc(A.un.p + ic * s0 + corner, plane_bytes);
+            rln = plane_rsrc(A.ln.p + ic * s0 + corner, fplane_bytes);
+            ro = plane_rsrc(A.out32 + ic * s0 + corner, fplane_bytes);
+        } else {
+            ro = plane_rsrc(A.out64 + ic * s0 + corner, plane_bytes);
+        }
+#pragma unroll
+        for (int t = 1; t <= R; t++) {
+            const bool out = lane_out && yout[t];
+            const double rv = hilo_residual_point(H0[t], Hm1[t], Hp1[t], H0[t - 1], H0[t + 1], lane_prev(H0[t]),
+                                                  lane_next(H0[t]), L0[t], Lm1[t], Lp1[t], L0[t - 1], L0[t + 1],
+                                                  lane_prev(L0[t]), lane_next(L0[t]), F0[t], A.inv_h2);
+            if (UPD) {
+                bstore(H0[t], rn, out ? zoff : BUF_OOB, roff[t]);
+                bstore(L0[t], rln, out ? zoff / 2 : BUF_OOB, roff[t] / 2);
+                bstore((float)rv, ro, out ? zoff / 2 : BUF_OOB, roff[t] / 2);
+            } else {
+                bstore(rv, ro, out ? zoff : BUF_OOB, roff[t]);
+            }
+            vmax = out ? fmax(vmax, fabs(rv)) : vmax;
+        }
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            Hm1[t] = H0[t];
+            H0[t] = Hp1[t];
+            Hp1[t] = HL[t];
+            Lm1[t] = L0[t];
+            L0[t] = Lp1[t];
+            Lp1[t] = LL[t];
+            F0[t] = FL[t];
+        }
+    }
+    if (A.norm) {
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) vmax = fmax(vmax, __shfl_xor(vmax, o, 64));
+        if (lane == 0 && vmax > 0.0) atomicMax(A.norm, (unsigned long long)__double_as_longlong(vmax));
+    }
+}
+
 // max |r| over a region (the part of the shape outside the box), into *norm
 template <class T>
 __global__ void k_norm_region(View<T> r, int64_t x0, int64_t x1, int64_t y0, int64_t y1, int64_t z0, int64_t z1,
@@ -1733,6 +1969,95 @@ int march_update_residual(int mode, const mgp_array* u, const mgp_array* un, con
     }
     if (norm) {
         // the shape points outside the box, whose r is read as stored
+        const int64_t n0 = u->shape[0], n1 = u->shape[1], n2 = u->shape[2];
+        struct Reg { int64_t x0, x1, y0, y1, z0, z1; };
+        const Reg regs[6] = {{0, b[0], 0, n1, 0, n2},       {b[1], n0, 0, n1, 0, n2},
+                             {b[0], b[1], 0, b[2], 0, n2},  {b[0], b[1], b[3], n1, 0, n2},
+                             {b[0], b[1], b[2], b[3], 0, b[4]}, {b[0], b[1], b[2], b[3], b[5], n2}};
+        for (const Reg& g : regs) {
+            const int64_t cnt = (g.x1 - g.x0) * (g.y1 - g.y0) * (g.z1 - g.z0);
+            if (g.x1 <= g.x0 || g.y1 <= g.y0 || g.z1 <= g.z0) continue;
+            const int blocks = (int)std::min<int64_t>(1024, (cnt + 255) / 256);
+            hipLaunchKernelGGL(k_norm_region<double>, dim3(blocks), dim3(256), 0, st, A.r, g.x0, g.x1, g.y0, g.y1,
+                               g.z0, g.z1, (unsigned long long*)norm);
+        }
+    }
+    const hipError_t err = hipGetLastError();
+    return err == hipSuccess ? MGP_OK : (int)err;
+}
+
+// rows per wave tile of the hi + lo update + residual: 8 or 16
+int64_t g_hilo_rows = 16;
+
+int march_hilo_residual(const mgp_array* u, const mgp_array* un, const mgp_array* lo, const mgp_array* ln,
+                        const mgp_array* e, const mgp_array* f, const mgp_array* r, const mgp_array* out,
+                        const mgp_faces* faces, const int64_t b[6], double* norm, void* stream) {
+    const bool upd = e != nullptr;
+    const mgp_array* fl[] = {lo, upd ? ln : lo, upd ? e : lo};
+    const mgp_array* dl[] = {un ? un : u, f, r};
+    for (int d = 0; d < 3; d++) {
+        for (const mgp_array* a : fl)
+            if (a->stride[d] != u->stride[d] || a->shape[d] != u->shape[d]) return MGP_ERR_ARG;
+        for (const mgp_array* a : dl)
+            if (a->stride[d] != u->stride[d] || a->shape[d] != u->shape[d]) return MGP_ERR_ARG;
+        if (out->stride[d] != u->stride[d] || out->shape[d] != u->shape[d]) return MGP_ERR_ARG;
+    }
+    for (int q = 0; q < 6; q++) {
+        for (const mgp_array* a : fl)
+            if (a->halo[q] != u->halo[q]) return MGP_ERR_ARG;
+        for (const mgp_array* a : dl)
+            if (a->halo[q] != u->halo[q]) return MGP_ERR_ARG;
+        if (out->halo[q] != u->halo[q]) return MGP_ERR_ARG;
+    }
+    HiloArgs A;
+    A.u = view_of<double>(*u);
+    A.un = view_of<double>(upd ? *un : *u);
+    A.f = view_of<double>(*f);
+    A.r = view_of<double>(*r);
+    A.lo = view_of<float>(*lo);
+    A.ln = view_of<float>(upd ? *ln : *lo);
+    A.e = view_of<float>(upd ? *e : *lo);
+    A.out64 = upd ? nullptr : view_of<double>(*out).p;
+    A.out32 = upd ? view_of<float>(*out).p : nullptr;
+    for (int q = 0; q < 6; q++)
+        A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const double*>(faces->g[q]) : nullptr;
+    A.two_h = 2.0 * faces->h;  // include/neumann.h:108
+    A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
+    A.xlo = -u->halo[MGP_WEST], A.xhi = u->shape[0] - 1 + u->halo[MGP_EAST];
+    A.ylo = -u->halo[MGP_SOUTH], A.yhi = u->shape[1] - 1 + u->halo[MGP_NORTH];
+    A.zlo = -u->halo[MGP_BOTTOM], A.zhi = u->shape[2] - 1 + u->halo[MGP_TOP];
+    const int64_t plane_bytes = ((A.yhi - A.ylo) * u->stride[1] + (A.zhi - A.zlo) + 1) * (int64_t)sizeof(double);
+    if (plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    A.inv_h2 = 1.0 / (faces->h * faces->h);
+    A.norm = (unsigned long long*)norm;
+    const int R = g_hilo_rows == 8 ? 8 : 16;
+    const int64_t nx = b[1] - b[0], ny = b[3] - b[2], nz = b[5] - b[4];
+    const hipStream_t st = (hipStream_t)stream;
+    A.ytiles = A.ztiles = A.ntiles = A.ntiles_pad = 0;
+    A.xchunk = 1;
+    if (nx > 0 && ny > 0 && nz > 0) {
+        A.ytiles = (int)((ny + R - 1) / R);
+        A.ztiles = (int)((nz + RZ - 1) / RZ);
+        const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
+        int64_t chunks = std::max<int64_t>(1, target_wgs(g_upd_target) / std::max<int64_t>(tiles, 1));
+        chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
+        A.xchunk = (nx + chunks - 1) / chunks;
+        A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
+        A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
+    }
+    const int copy = upd ? complement_blocks(A.u, b, 64) : 0;
+    if (A.ntiles_pad + copy > 0) {
+        const dim3 g(A.ntiles_pad + copy), blk(64);
+        if (upd) {
+            if (R == 8) hipLaunchKernelGGL((k_hilo_residual_wave<8, true>), g, blk, 0, st, A);
+            else hipLaunchKernelGGL((k_hilo_residual_wave<16, true>), g, blk, 0, st, A);
+        } else {
+            if (R == 8) hipLaunchKernelGGL((k_hilo_residual_wave<8, false>), g, blk, 0, st, A);
+            else hipLaunchKernelGGL((k_hilo_residual_wave<16, false>), g, blk, 0, st, A);
+        }
+    }
+    if (norm) {
+        // the shape points outside the box, whose r is read as stored (FP64 r)
         const int64_t n0 = u->shape[0], n1 = u->shape[1], n2 = u->shape[2];
         struct Reg { int64_t x0, x1, y0, y1, z0, z1; };
         const Reg regs[6] = {{0, b[0], 0, n1, 0, n2},       {b[1], n0, 0, n1, 0, n2},

@@ -19,7 +19,7 @@ REFERENCE, CONSISTENT = 0, 1
 INJECTION, FULL_WEIGHTING = 0, 1
 JACOBI, GAUSS_SEIDEL = 0, 1
 F64, F32, MIXED = 0, 1, 2
-NO_GRAPHS, NO_FUSED, QUIET = 1, 2, 4
+NO_GRAPHS, NO_FUSED, QUIET, PLAIN_ITERATE, COARSE_FP32 = 1, 2, 4, 8, 16
 OK, ERR_ARG, ERR_NODEV, ERR_STATE = 0, -1, -2, -3
 LOC = "NSEWTB"
 
@@ -90,6 +90,9 @@ _SIGS = {
     "mgp_add_f32_to_f64": [P(Array), P(Array), vp],
     "mgp_update_residual_f64_f32": [i32, P(Array), P(Array), P(Array), P(Array), P(Array), P(Array), P(Faces), BOX,
                                     vp, vp],
+    "mgp_update_residual_hilo_f64_f32": [P(Array), P(Array), P(Array), P(Array), P(Array), P(Array), P(Array),
+                                         P(Array), P(Faces), BOX, vp, vp],
+    "mgp_residual_hilo_f64": [P(Array), P(Array), P(Array), P(Array), P(Faces), BOX, vp, vp],
     "mgp_graph_begin": [vp], "mgp_graph_end": [vp, P(vp)], "mgp_graph_launch": [vp, vp], "mgp_graph_destroy": [vp],
     "mgp_event_create": [P(vp)], "mgp_event_destroy": [vp], "mgp_event_record": [vp, vp], "mgp_event_sync": [vp],
     "mgp_event_elapsed_ms": [vp, vp, P(f)],
@@ -110,6 +113,7 @@ _SIGS = {
     "mgp_server_stats": [vp, P(i64), P(i64), P(i64)],
     "mgp_fill_f16": [P(Array), d, vp], "mgp_rbgs_half_f16": [P(Array), P(Array), BOX, d, d, i32, vp],
     "mgp_rbgs_sweeps_f16": [P(Array), P(Array), BOX, d, d, i64, vp],
+    "mgp_rbgs_sweeps_f32_f64": [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
     "mgp_residual_f16": [P(Array), P(Array), P(Array), BOX, d, vp, vp],
     "mgp_restrict_fw_scaled_f16": [P(Array), P(Array), vp, vp],
     "mgp_restrict_fw_scaled_f32_f16": [P(Array), P(Array), vp, vp],
@@ -322,6 +326,17 @@ class Solver:
 
     def layout(self, level=0):
         return self._arr(level, "u")[1]
+
+    def lo_word(self):
+        """FP32 lo word of a mixed solver's hi + lo iterate (level 0, flat,
+        halos included); the iterate is array("u") + lo_word()."""
+        _, a = self._arr(0, "l")
+        n = a.stride[0] * (a.shape[0] + a.halo[2] + a.halo[3])
+        out = np.empty(n, np.float32)
+        self.sync()
+        check(lib().mgp_memcpy_d2h(out.ctypes.data, a.data, out.nbytes, None), "d2h")
+        check(lib().mgp_stream_sync(None), "sync")
+        return out
 
     def array(self, which, level=0):
         """Flat numpy view (shares memory) of the host mirror of u/p/f/r."""

@@ -79,6 +79,8 @@ public:
     // addition: coarser levels kept elsewhere (the last level of a hierarchy
     // only; nullptr otherwise)
     CoarseLevels<T>* below = nullptr;
+    // addition: the last level of its solver's hierarchy
+    bool coarsest = false;
 
     Domain() : halo(), is_initialized(false), requires_duplicate(true), settings() {}
     Domain(Settings& _settings, BoundaryCondition& BC, bool requires_duplicate, int_t device[3], int_t device_shape[3]);

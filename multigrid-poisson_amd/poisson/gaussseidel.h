@@ -11,6 +11,8 @@
 #ifndef POISSON_GAUSS_SEIDEL
 #define POISSON_GAUSS_SEIDEL
 
+#include <type_traits>
+
 #include "relaxation.h"
 
 namespace Poisson {
@@ -53,6 +55,14 @@ public:
             const mgp_array ui = d.u->desc();
             mgp_array uo = ui;
             if (nsmooth & 1) uo.data = d.u_alt;
+            // FP32 coarsest level of a refinement solve: the passes in FP64
+            // arithmetic where the level fits the LDS in doubles
+            const bool dp = std::is_same<T, float>::value && d.settings.coarse_fp64 && d.coarsest && !d.below;
+            if (dp && mgp_rbgs_sweeps_f32_f64(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega, nsmooth,
+                                              mgpx::stream()) == MGP_OK) {
+                if (nsmooth & 1) d.swap_u_alt();
+                return;
+            }
             if (mgpx::Ops<T>::rbgs_sweeps(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega, nsmooth,
                                           mgpx::stream()) == MGP_OK) {
                 if (nsmooth & 1) d.swap_u_alt();

@@ -210,6 +210,7 @@ void PoissonSolver<T, R, P, S>::alloc() {
         domains[0][l] = new Domain<T>(grid.domainsettings[l], BC, relaxation.requires_duplicate_solution(), device,
                                       dev_shape);
     }
+    domains[0][settings.levels - 1]->coarsest = true;
 }
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>

@@ -8,9 +8,19 @@
 //                                         homogeneous boundary data, all levels)
 //     u += e                            (FP64 update)
 // until ||r||inf / ||f||inf <= tolerance.  FP32 carries the smoothing,
-// restriction and prolongation; FP64 the residual, the update and the
-// convergence test, so the attainable accuracy is that of the FP64 residual.
-// It requires SolverMode CONSISTENT (homogeneous coarse Neumann data).
+// restriction and prolongation; FP64 the residual, the update, the
+// convergence test and (Settings::refine_coarse_fp64) the arithmetic of the
+// coarsest level's smoothing, so the attainable accuracy is that of the FP64
+// residual.  It requires SolverMode CONSISTENT (homogeneous coarse Neumann
+// data).
+//
+// Settings::refine_hilo (default) holds the iterate as FP64 hi plus an FP32
+// lo word (the update renormalised by an error-free sum, the residual taken
+// of hi + lo: mgp_update_residual_hilo_f64_f32).  A plain FP64 u floors
+// ||r||/||f|| near 12/h^2 ulp(u)/||f|| (about 1e-10 at h = 1/1024); hi + lo
+// carries about 77 bits, so the FP64 residual keeps falling.  get_solution()
+// is hi, the FP64 rounding of the iterate; its residual and relative_residual()
+// are those of hi + lo.
 //
 // Settings::half_from > 0 keeps levels half_from and coarser of the FP32
 // cycle in FP16 storage (Fp16Levels, fp16levels.h): the FP32 solver then has
@@ -18,6 +28,7 @@
 #ifndef POISSON_REFINEMENT
 #define POISSON_REFINEMENT
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <iomanip>
@@ -54,6 +65,7 @@ public:
         }
         inner_settings.print_cycle_time = false;
         outer_settings.print_cycle_time = false;
+        inner_settings.coarse_fp64 = s.refine_coarse_fp64;
         inner_settings.print_settings = false;  // the FP64 levels print, as one solver would
         outer = new PoissonSolver<double, R, P, S>(outer_settings, BC);
         try {
@@ -80,6 +92,7 @@ public:
         dev_norm = static_cast<double*>(p);
         mgpx::check(mgp_host_alloc(&p, 8 * sizeof(double)), "RefinementSolver");
         host_norm = static_cast<double*>(p);
+        if (s.refine_hilo) alloc_hilo();
     }
     RefinementSolver(const RefinementSolver&) = delete;
     RefinementSolver& operator=(const RefinementSolver&) = delete;
@@ -89,6 +102,8 @@ public:
         delete outer;
         mgp_free(dev_norm);
         mgp_host_free(host_norm);
+        for (void* q : lo) mgp_free(q);
+        mgp_free(own_alt);
     }
 
     void init() { this->init(&ufun, &ffun, &dudxfun, &dudyfun, &dudzfun); }
@@ -132,8 +147,9 @@ public:
                 if (iteration_done(since() > maxtime)) break;
             }
             // the iterations wrote r only as FP32: the FP64 r of the final u
+            // (of hi + lo: the residual the stop rule saw)
             mgpx::StreamScope scope(outer->get_stream());
-            residual<double>(outer->get_domain(0, 0), true);
+            final_residual();
             mgpx::check(mgp_stream_sync(outer->get_stream()), "refinement");
         }
         wtime = since();
@@ -153,9 +169,16 @@ public:
         mgpx::StreamScope scope(st);
         Domain<double>& d64 = outer->get_domain(0, 0);
         fnorm = d64.f->infinity_norm();
+        // a solve starts from the FP64 u as stored: lo = 0
+        if (lo[0]) {
+            const mgp_array l = lo_desc(cur);
+            mgpx::check(mgp_fill_f32(&l, 0.0, st), "refinement: lo");
+        }
         enqueue_norm(d64, st);
         mgpx::check(mgp_stream_sync(st), "refinement: norm");
         rel_res = host_norm[0] / fnorm;
+        best = rel_res;
+        stall = 0;
         iter = 0;
         f32_ready = false;
     }
@@ -174,8 +197,22 @@ public:
         inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
         const mgp_array e32 = d32.u->desc();
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
-        // update and residual in one pass where nothing runs between them
-        if (settings.refine_polish > 0 || !update_residual_norm_to_f32(d64, e32, f32, dev_norm)) {
+        if (lo[0]) {
+            // hi + lo iterate: the update renormalised into the second pair
+            // of buffers and the residual of it, in one pass
+            int64_t b[6];
+            d64.box(b);
+            const mgp_array f64 = d64.f->desc(), r64v = d64.r->desc(), l0 = lo_desc(cur), l1 = lo_desc(cur ^ 1);
+            mgp_array un = u64;
+            un.data = hi_alt();
+            const mgp_faces F = d64.faces();
+            mgpx::check(mgp_update_residual_hilo_f64_f32(&u64, &un, &l0, &l1, &e32, &f64, &r64v, &f32, &F, b, dev_norm,
+                                                         st),
+                        "refinement: update + residual");
+            swap_hi();
+            cur ^= 1;
+        } else if (settings.refine_polish > 0 || !update_residual_norm_to_f32(d64, e32, f32, dev_norm)) {
+            // update and residual in one pass where nothing runs between them
             mgpx::check(mgp_add_f32_to_f64(&u64, &e32, st), "refinement: update");
             // optional FP64 smoothing of the updated solution: settles u at a
             // fixed point of the FP64 smoother, whose residual sits below the
@@ -186,20 +223,53 @@ public:
         mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
         f32_ready = true;
     }
-    // stop rule of solve(); `timed_out` stands for the wall-time limit
+    // stop rule of solve(); `timed_out` stands for the wall-time limit.
+    // Consistent-mode rule: tolerance reached, divergence (norm > 2x the
+    // previous), wall time, maxiter, or stagnation: no new smallest norm in
+    // the last `stall_limit` iterations (a floor the iteration cannot pass;
+    // the solve then ends with the tolerance not reached)
+    static constexpr int stall_limit = 4;
     bool iteration_done(bool timed_out = false) {
         const double norm = host_norm[0] / fnorm;
         if (norm <= settings.tolerance) {
             rel_res = norm;
             return true;
         }
-        if (iter >= settings.miniter && (norm > 2.0 * rel_res || timed_out)) {
+        if (norm < best) {
+            best = norm;
+            stall = 0;
+        } else {
+            stall++;
+        }
+        if (iter >= settings.miniter && (norm > 2.0 * rel_res || timed_out || stall >= stall_limit)) {
             rel_res = norm;
             return true;
         }
         rel_res = norm;
         iter++;
         return iter >= settings.maxiter;
+    }
+    // the final residual of a solve driven through start()/enqueue_iteration()
+    // (solve() calls it itself): the FP64 r of the final iterate, enqueued
+    void final_residual() {
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        if (!lo[0]) {
+            residual<double>(d64, true);
+            return;
+        }
+        int64_t b[6];
+        d64.box(b);
+        const mgp_array u = d64.u->desc(), f = d64.f->desc(), r = d64.r->desc(), l = lo_desc(cur);
+        const mgp_faces F = d64.faces();
+        mgpx::check(mgp_residual_hilo_f64(&u, &l, &f, &r, &F, b, nullptr, mgpx::stream()), "refinement: residual");
+    }
+    // device descriptor of the FP32 lo word of the iterate (level 0's layout);
+    // data NULL where the iterate is plain FP64
+    mgp_array solution_lo() const {
+        if (lo[0]) return lo_desc(cur);
+        mgp_array a = outer->get_domain(0, 0).u->desc();
+        a.data = nullptr;
+        return a;
     }
     double relative_residual() { return rel_res; }
     double solve_time() { return wtime; }
@@ -217,8 +287,48 @@ public:
 private:
     void enqueue_norm(Domain<double>& d, void* st) {
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
-        residual_norm<double>(d, dev_norm);
+        if (lo[0]) {
+            int64_t b[6];
+            d.box(b);
+            const mgp_array u = d.u->desc(), f = d.f->desc(), r = d.r->desc(), l = lo_desc(cur);
+            const mgp_faces F = d.faces();
+            mgpx::check(mgp_residual_hilo_f64(&u, &l, &f, &r, &F, b, dev_norm, st), "refinement: residual");
+        } else {
+            residual_norm<double>(d, dev_norm);
+        }
         mgpx::check(mgp_memcpy_d2h(host_norm, dev_norm, sizeof(double), st), "refinement: norm");
+    }
+    // the FP32 lo words (two buffers: the update writes the other one) and,
+    // where the FP64 level has no second u buffer, one of its own
+    void alloc_hilo() {
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        const size_t n = (size_t)d64.u->size;
+        mgpx::StreamScope scope(outer->get_stream());
+        for (int k = 0; k < 2; k++) {
+            mgpx::check(mgp_malloc(&lo[k], n * sizeof(float)), "RefinementSolver: lo");
+            const mgp_array l = lo_desc(k);
+            mgpx::check(mgp_fill_f32(&l, 0.0, outer->get_stream()), "RefinementSolver: lo");
+        }
+        if (!d64.u_alt) {
+            void* p = nullptr;
+            mgpx::check(mgp_malloc(&p, n * sizeof(double)), "RefinementSolver: u");
+            own_alt = static_cast<double*>(p);
+        }
+        mgpx::check(mgp_stream_sync(outer->get_stream()), "RefinementSolver: lo");
+    }
+    mgp_array lo_desc(int k) const {
+        mgp_array a = outer->get_domain(0, 0).u->desc();
+        a.data = lo[k];
+        return a;
+    }
+    void* hi_alt() { return own_alt ? (void*)own_alt : (void*)outer->get_domain(0, 0).u_alt; }
+    void swap_hi() {
+        Domain<double>& d64 = outer->get_domain(0, 0);
+        if (own_alt) {
+            std::swap(d64.u->devptr, own_alt);
+        } else {
+            d64.swap_u_alt();
+        }
     }
 
     Settings settings, outer_settings, inner_settings;
@@ -231,8 +341,12 @@ private:
     bool is_verbose = false;
     bool fcycle = true;
     bool f32_ready = false;  // the FP32 right-hand side already holds (float) r
+    void* lo[2] = {nullptr, nullptr};  // FP32 lo word of the hi + lo iterate (ping-pong)
+    double* own_alt = nullptr;         // second hi buffer where level 0 has no u_alt
+    int cur = 0;                       // lo[cur] belongs to the current u
     uint_t iter = 0;
-    double rel_res = 0.0, wtime = 0.0, fnorm = 1.0;
+    int stall = 0;
+    double rel_res = 0.0, wtime = 0.0, fnorm = 1.0, best = 0.0;
 };
 }  // namespace Poisson
 #endif

@@ -60,6 +60,12 @@ public:
     uint_t half_from = 0;          // RefinementSolver: levels >= half_from of the FP32 cycle stored
                                    // in FP16 (0 = none; Gauss-Seidel, full weighting, all faces
                                    // Dirichlet)
+    bool refine_hilo = true;       // RefinementSolver: the iterate held as FP64 hi + FP32 lo, so
+                                   // that its FP64 residual is not floored by u's rounding to FP64
+    bool refine_coarse_fp64 = true;  // RefinementSolver: the FP32 cycle's coarsest level smoothed
+                                     // with FP64 arithmetic (sets coarse_fp64 of its FP32 levels)
+    bool coarse_fp64 = false;      // FP32 levels: the coarsest level's Gauss-Seidel passes in FP64
+                                   // arithmetic where the level fits one workgroup's LDS
 
     Settings() {}
     Settings(const Settings& s) = default;

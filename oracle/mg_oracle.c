@@ -856,3 +856,91 @@ void og_f2h_n(const float* x, uint16_t* out, int64_t n) {
 void og_h2f_n(const uint16_t* x, float* out, int64_t n) {
     for (int64_t t = 0; t < n; t++) out[t] = og_h2f(x[t]);
 }
+
+/* ------------------------------------------------ hi + lo refinement iterate
+ * (this project's addition; no counterpart in the reference) */
+
+/* Knuth's TwoSum (The Art of Computer Programming vol. 2, 4.2.2):
+ * s = fl(a + b), a + b = s + *e exactly (built with -ffp-contract=off) */
+static double og_two_sum(double a, double b, double* e) {
+    const double s = a + b;
+    const double bb = s - a;
+    *e = (a - (s - bb)) + (b - bb);
+    return s;
+}
+
+void og_hilo_update(const og_layout* L, double* hi, float* lo, const float* e) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < L->n[0]; i++)
+        for (int64_t j = 0; j < L->n[1]; j++)
+            for (int64_t k = 0; k < L->n[2]; k++) {
+                const int64_t c = IX(L, i, j, k);
+                double err;
+                hi[c] = og_two_sum(hi[c], (double)e[c] + (double)lo[c], &err);
+                lo[c] = (float)err;
+            }
+}
+
+/* (hi, lo) of the neighbour of in-shape point (i,j,k) one step along `axis`
+ * in direction `dir`: stored, or the Neumann ghost formed from the mirror */
+static void og_hilo_nb(const og_layout* L, const double* hi, const float* lo, const double* const g[6], double two_h,
+                       int64_t i, int64_t j, int64_t k, int axis, int dir, double* vh, float* vl) {
+    int64_t a[3] = {i, j, k};
+    a[axis] += dir;
+    if (a[axis] >= 0 && a[axis] < L->n[axis]) {
+        const int64_t c = IX(L, a[0], a[1], a[2]);
+        *vh = hi[c];
+        *vl = lo[c];
+        return;
+    }
+    int face;
+    int64_t gi;
+    if (axis == 0) {
+        face = dir > 0 ? OG_E : OG_W;
+        gi = j * L->n[2] + k; /* (1, ny, nz) */
+    } else if (axis == 1) {
+        face = dir > 0 ? OG_N : OG_S;
+        gi = i * L->n[2] + k; /* (nx, 1, nz) */
+    } else {
+        face = dir > 0 ? OG_T : OG_B;
+        gi = i * L->n[1] + j; /* (nx, ny, 1) */
+    }
+    a[axis] -= 2 * dir; /* mirror */
+    const int64_t m = IX(L, a[0], a[1], a[2]);
+    const double p = (dir > 0 ? 1.0 : -1.0) * two_h * g[face][gi];
+    double err;
+    *vh = og_two_sum(p, hi[m], &err);
+    *vl = (float)((double)lo[m] + err);
+}
+
+double og_hilo_residual(const og_layout* L, const double* hi, const float* lo, const double* f, double* r,
+                        const double* const g[6], const int64_t box[6], double h) {
+    const double two_h = 2.0 * h, inv_h2 = 1.0 / (h * h);
+    double vmax = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : vmax)
+    for (int64_t i = box[0]; i < box[1]; i++)
+        for (int64_t j = box[2]; j < box[3]; j++)
+            for (int64_t k = box[4]; k < box[5]; k++) {
+                const int64_t c = IX(L, i, j, k);
+                double nh[6];
+                float nl[6];
+                /* W, E, S, N, B, T */
+                for (int q = 0; q < 6; q++)
+                    og_hilo_nb(L, hi, lo, g, two_h, i, j, k, q / 2, (q & 1) ? 1 : -1, &nh[q], &nl[q]);
+                const double hc = hi[c], lc = (double)lo[c];
+                const double sh = ((nh[0] - hc) + (nh[1] - hc)) + ((nh[2] - hc) + (nh[3] - hc)) +
+                                  ((nh[4] - hc) + (nh[5] - hc));
+                const double sl = (((double)nl[0] - lc) + ((double)nl[1] - lc)) +
+                                  (((double)nl[2] - lc) + ((double)nl[3] - lc)) +
+                                  (((double)nl[4] - lc) + ((double)nl[5] - lc));
+                r[c] = fma(-inv_h2, sh + sl, f[c]);
+                vmax = MAX(vmax, fabs(r[c]));
+            }
+    for (int64_t i = 0; i < L->n[0]; i++)
+        for (int64_t j = 0; j < L->n[1]; j++)
+            for (int64_t k = 0; k < L->n[2]; k++) {
+                if (i >= box[0] && i < box[1] && j >= box[2] && j < box[3] && k >= box[4] && k < box[5]) continue;
+                vmax = MAX(vmax, fabs(r[IX(L, i, j, k)]));
+            }
+    return vmax;
+}

@@ -146,6 +146,25 @@ void og_h_restrict_fw(const og_layout* Li, const void* in, int in_f32, const og_
 void og_h_prolong_add(const og_layout* Lc, const uint16_t* coarse, const og_layout* Lf, void* fine, int fine_f32,
                       double norm);
 
+/* ---- the iterate of mixed-precision refinement held as hi + lo (FP64 hi,
+ * FP32 lo; this project's addition, no counterpart in the reference).
+ * Consistent mode only.  Update: hi' = fl(hi + fl(e + lo)) by Knuth's
+ * error-free TwoSum, lo' = (float) of its rounding error.  Residual of hi + lo:
+ * r = fma(-1/h^2, S(hi) + S(lo), f) with S the consistent-mode sum of
+ * neighbour differences (x pair, y pair, z pair), the lo differences in FP64.
+ * A neighbour beyond a Neumann face is the ghost p + mirror, p =
+ * (sign*2h)*g as include/neumann.h:112-130 forms it, hi = fl(p + hi(mirror))
+ * and lo = (float)(lo(mirror) + rounding error of that sum).
+ * Pinned by: exact rational arithmetic (Python fractions) on inputs where
+ * every FP64 operation is exact, and by the TwoSum identity hi' + err =
+ * hi + fl(e + lo) on random inputs.
+ * g[q]: face data of Neumann faces (layouts as in mgp.h), NULL elsewhere. */
+void og_hilo_update(const og_layout* L, double* hi, float* lo, const float* e);
+/* r on the box (r elsewhere untouched); returns max |r| over the box and r's
+ * stored values outside it */
+double og_hilo_residual(const og_layout* L, const double* hi, const float* lo, const double* f, double* r,
+                        const double* const g[6], const int64_t box[6], double h);
+
 #ifdef __cplusplus
 }
 #endif

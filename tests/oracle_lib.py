@@ -89,6 +89,9 @@ def lib():
         L.og_h_residual.restype = d
         L.og_h_restrict_fw.argtypes = [P(Layout), vp, i32, P(Layout), vp, d]
         L.og_h_prolong_add.argtypes = [P(Layout), vp, P(Layout), vp, i32, d]
+        L.og_hilo_update.argtypes = [P(Layout), vp, vp, vp]
+        L.og_hilo_residual.argtypes = [P(Layout), vp, vp, vp, vp, P(vp * 6), P(i64 * 6), d]
+        L.og_hilo_residual.restype = d
         _lib = L
     return _lib
 
@@ -248,3 +251,23 @@ def h_prolong_add(coarse, fine, norm):
     """fine (float16 or float32) += P(coarse) / sigma(norm), in place."""
     lib().og_h_prolong_add(ctypes.byref(layout(coarse.shape)), _hp(coarse), ctypes.byref(layout(fine.shape)),
                            _hp(fine, (np.float16, np.float32)), int(fine.dtype == np.float32), float(norm))
+
+
+# ---- the hi + lo iterate of mixed-precision refinement (FP64 hi, FP32 lo)
+
+def _fp(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def hilo_update(L, hi, lo, e):
+    """In place: hi, lo (flat padded float64 / float32) += e (float32)."""
+    lib().og_hilo_update(ctypes.byref(L), ptr(hi), _fp(lo), _fp(e))
+
+
+def hilo_residual(L, hi, lo, f, r, faces, box, h):
+    """r = f - L(hi + lo) on the box; faces: dict face -> float64 face array
+    for Neumann faces.  Returns max |r| over the box and r outside it."""
+    g = (ctypes.c_void_p * 6)(*[ptr(faces[q]) if q in faces else None for q in range(6)])
+    return lib().og_hilo_residual(ctypes.byref(L), ptr(hi), _fp(lo), ptr(f), ptr(r), ctypes.byref(g),
+                                  ctypes.byref(box_arr(box)), float(h))

@@ -1,7 +1,9 @@
 """Batched solves (mgp_batch_*) on the GPU: every problem of a batch ends with
 the same bits (solution, iteration count, relative residual) as the same
-problem solved alone by the single-problem mixed-precision solver, which the
-oracle pins through the FP64 path (tests/test_gpu_parity.py)."""
+problem solved alone by the single-problem mixed-precision solver with the
+batch's arithmetic (plain FP64 iterate, FP32 coarsest level:
+MGP_SOLVER_PLAIN_ITERATE | MGP_SOLVER_COARSE_FP32).  The batch against the
+oracle's FP64 solves: tests/test_gpu_refinement.py."""
 import ctypes
 import zlib
 
@@ -42,7 +44,7 @@ def fourier_rhs(d, rng):
 def single_solve(d, f, relaxation, nsmooth, omega, coarse_sweeps, maxiter, tol):
     s = mgp.Solver(d, max_levels(d), 1.0 / (d[0] - 1), (0.0, 0.0, 0.0), "DDDDDD", mgp.CONSISTENT, relaxation,
                    mgp.FULL_WEIGHTING, problem=-1, dtype=mgp.MIXED, maxiter=maxiter, miniter=1, tolerance=tol,
-                   coarse_sweeps=coarse_sweeps)
+                   coarse_sweeps=coarse_sweeps, flags=mgp.QUIET | mgp.PLAIN_ITERATE | mgp.COARSE_FP32)
     s.array("f")[:] = f.ravel()
     s.to_device()
     it, rel, _ = s.solve("Vcycle", nsmooth, omega)
