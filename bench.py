@@ -171,9 +171,13 @@ def main():
     batch = guarded_leg("batch", args.no_batch, lambda: batch_leg(args))
     serve = guarded_leg("serve", args.no_serve, lambda: serve_leg(args))
 
-    cpu = None
+    cpu = port = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads, args.coarse_sweeps)
+        port = cpu_baseline(dims, args.levels, h, args.nsmooth, cycles[-1], args.cpu_threads, args.coarse_sweeps)
+        cpu = guarded_leg("cpu reference", False,
+                          lambda: cpu_baseline_reference(dims, args.levels, args.nsmooth, cycles[-1], args.cpu_threads))
+        if not cpu or "error" in cpu:
+            cpu, port = port, cpu
 
     out = {
         "metric": "DOF/s to ||r||/||f|| <= 1e-10 (plain FP64 F-cycle, 1025x1025x513)",
@@ -202,6 +206,7 @@ def main():
         "batched": batch,
         "serving": serve,
         "cpu_baseline": cpu,
+        "cpu_port": port,
     }
     print(json.dumps(out), flush=True)
 
@@ -509,6 +514,72 @@ def serve_leg(args):
             "capacity_vs_fp64_one_at_a_time": burst["solves/s"] / seq_rate}
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def run_reference(params, threads, timeout):
+    """oracle/_ref/ref_timing (the reference's own sources, built with its own
+    host flags -O3 -ffast-math -fopenmp) in timing mode; returns the per-cycle
+    times the reference prints and its set-up / solve wall times."""
+    import subprocess
+    import tempfile
+
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_timing")
+    with tempfile.TemporaryDirectory() as tmp:
+        par = os.path.join(tmp, "p.txt")
+        with open(par, "w") as fp:
+            fp.write("".join("%s %s\n" % kv for kv in params.items()))
+        env = dict(os.environ, OMP_NUM_THREADS=str(threads), OMP_PROC_BIND="close", OMP_PLACES="cores")
+        out = subprocess.run([exe, par, tmp, tmp], capture_output=True, text=True, timeout=timeout, env=env,
+                             check=True).stdout
+    cyc = [float(l.split()[2]) for l in out.splitlines() if l.endswith(" seconds") and " took " in l]
+    tail = [l for l in out.splitlines() if l.startswith("harness ")][-1].split()
+    kv = dict(zip(tail[1::2], tail[2::2]))
+    return cyc, float(kv["setup_s"]), float(kv["solve_s"]), float(kv["rel_res"])
+
+
+def cpu_baseline_reference(dims, levels, nsmooth, cycles, threads):
+    """The reference's own CPU path timed on the host cores (OpenMP target
+    regions run on the host): one F-cycle of the headline workload's shape
+    (1025x1025x513, 7 levels, red-black GS omega 1, 2 sweeps, full weighting,
+    Neumann x/y and Dirichlet z requested, u = x^3 y^2 z) in the reference's own
+    boundary treatment (north/south and bottom forced Dirichlet, stalls at
+    O(h): it never reaches 1e-10), scaled to the GPU's cycle count; plus the
+    10 V-cycles of the 65^3 configuration in full."""
+    exe = os.path.join(ROOT, "oracle", "_ref", "ref_timing")
+    if not os.path.exists(exe):
+        return None
+    if threads <= 0:
+        threads = len(os.sched_getaffinity(0))
+    log("bench: CPU baseline (reference, %d threads): 65^3 V-cycles" % threads)
+    c1 = dict(mode="time", nx=65, ny=65, nz=65, levels=4, bc="DDDDDD", problem=0, restriction="inj",
+              relaxation="gs", cycle="Vcycle", nsmooth=2, omega=1.0, ncycles=10, lengthx=1.0)
+    cyc1, set1, sol1, rr1 = run_reference(c1, threads, 300)
+    log("bench: CPU baseline (reference): %dx%dx%d F-cycle" % dims)
+    c2 = dict(mode="time", nx=dims[0], ny=dims[1], nz=dims[2], levels=levels, bc="NNNNDD", problem=2,
+              restriction="fw", relaxation="gs", cycle="Fcycle", nsmooth=nsmooth, omega=1.0, ncycles=1, lengthx=1.0)
+    cyc2, set2, sol2, rr2 = run_reference(c2, threads, 900)
+    spc = sum(cyc2) / len(cyc2)
+    dof = dims[0] * dims[1] * dims[2]
+    log("bench: CPU baseline (reference) %.2f s per F-cycle (set-up %.1f s)" % (spc, set2))
+    return {"value": dof / (spc * cycles), "unit": "DOF/s", "cores": threads, "kind": "reference",
+            "sample": "the reference's own F-cycle at %dx%dx%d (%d levels, GS(%d), FW), 1 cycle of %.1f s as it "
+                      "prints it, scaled to the GPU's %d cycles to 1e-10; set-up %.1f s not counted"
+                      % (dims + (levels, nsmooth, spc, cycles, set2)),
+            "s_per_cycle": spc, "cpu_model": cpu_model(),
+            "flags": "g++ -O3 -ffast-math -fopenmp (the reference's config/g++.mkf host flags)",
+            "config_65cube_vcycles": {"cycles": len(cyc1), "s_per_cycle": sum(cyc1) / max(len(cyc1), 1),
+                                      "solve_s": sol1, "final_rel_res": rr1,
+                                      "DOF*cycles/s": 65 ** 3 * len(cyc1) / max(sol1, 1e-9)}}
+
+
 def cpu_baseline(dims, levels, h, nsmooth, cycles, threads, coarse_sweeps):
     """The oracle (this build's C restatement, OpenMP) on the host cores: one
     F-cycle plus the residual check at the full grid, scaled to the GPU's
@@ -529,7 +600,7 @@ def cpu_baseline(dims, levels, h, nsmooth, cycles, threads, coarse_sweeps):
     log("bench: CPU baseline %.2f s per F-cycle" % t)
     return {"value": dof / (t * cycles), "unit": "DOF/s", "cores": threads, "kind": "port",
             "sample": "1 F-cycle + residual at %dx%dx%d (%.1f s), scaled to %d cycles" % (dims + (t, cycles)),
-            "s_per_cycle": t}
+            "s_per_cycle": t, "cpu_model": cpu_model()}
 
 
 if __name__ == "__main__":

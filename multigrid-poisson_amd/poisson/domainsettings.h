@@ -1,9 +1,13 @@
-// Settings of one multigrid level (reference include/domainsettings.h,
-// src/domainsettings.cpp:12-29): dims (n >> l) + 1, spacing h * 2^l.
+// Settings of one multigrid level (the reference's DomainSettings:
+// include/domainsettings.h, src/domainsettings.cpp:12-29): the coarsening rule
+// dims (n >> l) + 1, spacing h 2^l, and the error text are the reference's
+// interface; the code is this library's.
 #ifndef DOMAIN_SETTINGS
 #define DOMAIN_SETTINGS
 
+#include <cmath>
 #include <iostream>
+#include <sstream>
 #include <stdexcept>
 #include <string>
 
@@ -15,33 +19,34 @@ class DomainSettings : public Settings {
 public:
     DomainSettings() : Settings() {}
 
-    DomainSettings(const Settings& _settings, const uint_t l) : Settings(_settings) {
-        if (l > 0) {
-            uint_t factor = 1;
-            uint_t original_dims[3] = {this->dims[0], this->dims[1], this->dims[2]};
-            this->dims[0] = (this->dims[0] >> l) + 1;
-            this->dims[1] = (this->dims[1] >> l) + 1;
-            this->dims[2] = (this->dims[2] >> l) + 1;
-            this->h *= (double_t)(factor << l);
-            if ((((this->dims[0] - 1) << l) + 1 != original_dims[0]) ||
-                (((this->dims[1] - 1) << l) + 1 != original_dims[1]) ||
-                (((this->dims[2] - 1) << l) + 1 != original_dims[2])) {
-                std::string errormsg = "Invalid grid of size: (" + std::to_string(this->dims[0]) + ",";
-                errormsg += std::to_string(this->dims[1]) + "," + std::to_string(this->dims[2]) + ")";
-                throw std::invalid_argument(errormsg);
-            }
+    // level l of a hierarchy over `fine`: every axis keeps (n >> l) + 1 points
+    // and the spacing grows to h 2^l (exact: a power-of-two scale); the fine
+    // axis must hold 2^l m + 1 points so that each coarse point sits on a
+    // fine one, else std::invalid_argument names the coarse size
+    DomainSettings(const Settings& fine, const uint_t l) : Settings(fine) {
+        if (l == 0) return;
+        const uint_t step = (uint_t)1 << l;
+        bool nested = true;
+        for (int axis = 0; axis < 3; axis++) {
+            nested = nested && fine.dims[axis] >= 1 && (fine.dims[axis] - 1) % step == 0;
+            dims[axis] = (fine.dims[axis] >> l) + 1;
+        }
+        h = std::ldexp(fine.h, (int)l);
+        if (!nested) {
+            std::ostringstream msg;
+            msg << "Invalid grid of size: (" << dims[0] << "," << dims[1] << "," << dims[2] << ")";
+            throw std::invalid_argument(msg.str());
         }
     }
 
     ~DomainSettings() {}
 };
 
-inline std::ostream& operator<<(std::ostream& os, const DomainSettings& settings) {
-    os << "DomainSettings:" << std::endl;
-    os << "\tDomain size: (" << settings.dims[0] << "," << settings.dims[1] << "," << settings.dims[2] << ")"
-       << std::endl;
-    os << "\tGridSpacing: " << settings.h << std::endl;
-    return os;
+// the level listing a solver prints at construction (the reference's text)
+inline std::ostream& operator<<(std::ostream& os, const DomainSettings& level) {
+    const uint_t* n = level.dims;
+    return os << "DomainSettings:\n\tDomain size: (" << n[0] << "," << n[1] << "," << n[2] << ")\n"
+              << "\tGridSpacing: " << level.h << std::endl;
 }
 }  // namespace Poisson
 #endif

@@ -14,6 +14,7 @@
 //    -DPROBLEMn in src/problem_definition.cpp); a run-time id picks one.
 //
 // Usage:  ref_harness <params-file> <in-dir> <out-dir>
+//         (mode "time": the reference's own solve timed; in-dir and out-dir unused)
 // The params file is "key value" lines; arrays are raw little-endian buffers in
 // the reference's own halo-padded layout (Array<T>::at, include/array.h:84-94).
 #include "libpoisson.h"
@@ -385,6 +386,30 @@ int run_vtk(const Params& p, const std::string& out) {
     return 0;
 }
 
+// Timing of the reference's own solve on the host cores (the CPU baseline):
+// one solve() with maxiter = miniter = ncycles and tolerance 0, so that it
+// runs exactly ncycles cycles (each followed by the residual and its norm,
+// libpoisson.h:254-353).  The reference prints "<cycle> took <s> seconds"
+// per cycle itself; the harness adds the set-up and solve wall times.
+template <class T, template <class> class R, template <class> class S>
+int run_time(const Params& p) {
+    Settings settings = make_settings(p);
+    settings.maxiter = p.i("ncycles", 1);
+    settings.miniter = settings.maxiter;
+    settings.tolerance = 0.0;
+    BoundaryCondition bc = parse_bc(p.s("bc", "DDDDDD"));
+    const double t0 = omp_get_wtime();
+    PoissonSolver<T, R, TrilinearInterpolation, S> solver(settings, bc);
+    solver.init();
+    solver.to_device();
+    const double t1 = omp_get_wtime();
+    solver.solve(p.s("cycle", "Fcycle"), p.i("nsmooth", 2), (T)p.f("omega", 1.0));
+    const double t2 = omp_get_wtime();
+    printf("harness setup_s %.6f solve_s %.6f iterations %.0f rel_res %.17g threads %d\n", t1 - t0, t2 - t1,
+           (double)solver.solve_iterations(), (double)solver.relative_residual(), omp_get_max_threads());
+    return 0;
+}
+
 }  // namespace
 
 int main(int argc, char** argv) {
@@ -401,6 +426,11 @@ int main(int argc, char** argv) {
     if (mode == "vtk") return run_vtk(p, out);
     const bool fw = p.s("restriction", "inj") == "fw";
     const bool gs = p.s("relaxation", "gs") == "gs";
+    if (mode == "time") {
+        if (!dbl) return 2;
+        if (fw) return gs ? run_time<double, FullWeighting, GaussSeidel>(p) : run_time<double, FullWeighting, Jacobi>(p);
+        return gs ? run_time<double, Injection, GaussSeidel>(p) : run_time<double, Injection, Jacobi>(p);
+    }
     if (dbl) {
         if (fw) return gs ? run_solve<double, FullWeighting, GaussSeidel>(p, out) : run_solve<double, FullWeighting, Jacobi>(p, out);
         return gs ? run_solve<double, Injection, GaussSeidel>(p, out) : run_solve<double, Injection, Jacobi>(p, out);

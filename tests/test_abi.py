@@ -93,3 +93,46 @@ def test_reference_drivers_compile_against_the_drop_in_headers(driver, tmp_path)
     assert r1.returncode == 0, r1.stderr[-3000:]
     r = _gxx(src, str(tmp_path / driver), ["-fopenmp", "-DPROBLEM2", str(obj)])
     assert r.returncode == 0, r.stderr[-4000:]
+
+
+def test_level_settings_follow_the_coarsening_rule(tmp_path):
+    """DomainSettings / Grid (the reference's src/domainsettings.cpp:12-29
+    rule): level l of n points per axis has (n >> l) + 1 points and spacing
+    h 2^l; an axis that is not 2^l m + 1 throws invalid_argument naming the
+    coarse size, as the reference's message does."""
+    src = tmp_path / "g.cpp"
+    src.write_text(r'''
+#include <cstdio>
+#include "grid.h"
+using namespace Poisson;
+int main() {
+    Settings s; s.dims[0] = 1025; s.dims[1] = 513; s.dims[2] = 33; s.h = 1.0 / 1024;
+    Grid g(s, 6, false);
+    for (int l = 0; l < 6; l++)
+        std::printf("%ld %ld %ld %.17g\n", g.domainsettings[l].dims[0], g.domainsettings[l].dims[1],
+                    g.domainsettings[l].dims[2], g.domainsettings[l].h);
+    s.dims[2] = 35;
+    try { DomainSettings d(s, 2); std::printf("no throw\n"); }
+    catch (const std::invalid_argument& e) { std::printf("%s\n", e.what()); }
+    s.dims[2] = 34;
+    try { DomainSettings d(s, 1); std::printf("no throw\n"); }
+    catch (const std::invalid_argument& e) { std::printf("%s\n", e.what()); }
+    s.dims[2] = 37;
+    try { DomainSettings d(s, 2); std::printf("no throw\n"); }
+    catch (const std::invalid_argument& e) { std::printf("%s\n", e.what()); }
+    return 0;
+}
+''')
+    exe = tmp_path / "g"
+    r = subprocess.run(["g++", "-std=c++17", "-I" + os.path.join(ROOT, "include"), "-I" + os.path.join(PKG, "poisson"),
+                        str(src), "-o", str(exe)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr[-3000:]
+    out = subprocess.run([str(exe)], capture_output=True, text=True).stdout.splitlines()
+    want = []
+    for l in range(6):
+        n = [(m >> l) + 1 if l else m for m in (1025, 513, 33)]
+        want.append("%d %d %d %.17g" % (n[0], n[1], n[2], 2.0 ** l / 1024))
+    assert out[:6] == want
+    assert out[6] == "Invalid grid of size: (257,129,9)"
+    assert out[7] == "Invalid grid of size: (513,257,18)"
+    assert out[8] == "no throw"
