@@ -675,8 +675,20 @@ def test_two_sweep_pass_rejects_in_place_and_thin_levels():
                                  mgp.box([1, 16, 1, 16, 1, 16]), 1.0, None) == mgp.ERR_ARG
     thin = (2, 17, 17)
     a, b, f = (mgp.DeviceArray(thin, halo) for _ in range(3))
-    assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F),
-                                 mgp.box([1, 1, 1, 16, 1, 16]), 1.0, None) == mgp.OK  # empty box
+    # an empty box out of place would leave the second buffer without the copy
+    # of the first: rejected (in place it is a no-op for the one-pass and the
+    # LDS sweeps)
+    empty = mgp.box([1, 1, 1, 16, 1, 16])
+    assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), empty, 1.0,
+                                 None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), empty, 1.0,
+                                None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, a.ref(), a.ref(), f.ref(), ctypes.byref(F), empty, 1.0,
+                                None) == mgp.OK
+    assert L.mgp_rbgs_sweeps_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), empty, 1.0, 3,
+                                 None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweeps_f64(mgp.CONSISTENT, a.ref(), a.ref(), f.ref(), ctypes.byref(F), empty, 1.0, 3,
+                                 None) == mgp.OK
     for q in (2, 3):
         F.type[q] = mgp.NEUMANN
     assert L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F),
