@@ -500,6 +500,30 @@ int mgp_server_submit(void* server, const mgp_request* request, int64_t* id);
 int mgp_server_wait(void* server, int64_t id, mgp_result* result);
 int mgp_server_stats(void* server, int64_t* completed, int64_t* peak_active, int64_t* bytes_resident);
 
+/* ------------------------------------------------ cycles without allocation
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ncycles V- or F-cycles ("Vcycle"/"vcycle"/"Fcycle"/
+ * "fcycle") on the caller's finest-level u and f, the values that ncycles
+ * mgp_solver_cycle calls give on a solver created with dtype MGP_F64, mode
+ * MGP_MODE_CONSISTENT, MGP_GAUSS_SEIDEL, MGP_FULL_WEIGHTING and the same dims,
+ * levels, coarse_sweeps, u, f and faces (omega < 0: 8/9, as that solver).  u
+ * and f have that solver's level-0 layout (one halo layer on each Neumann face,
+ * none on Dirichlet faces, dense strides); faces gives the face types, the
+ * Neumann data (required) and h of level 0.  The finest level's residual and
+ * second u buffer and every coarser level live in one library-owned device
+ * array of MGP_NOALLOC_SCRATCH_BYTES per GPU, held until the process ends: the
+ * call allocates nothing and does not synchronise.  With norm != NULL, *norm
+ * (a device double) = max |f - L u| over the box of the final u (written, not
+ * accumulated).  MGP_ERR_ARG when the hierarchy does not fit the array (level
+ * 0 of about 2^21 points or more) or the arguments do not fit the above.  Every
+ * call clears the parts of the array it reads before use, but calls that
+ * overlap on one GPU share the array and race. */
+#define MGP_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+int mgp_cycle_noalloc_f64(const char* cycle, const mgp_array* u, const mgp_array* f, const mgp_faces* faces,
+                          int64_t levels, int64_t ncycles, int64_t nsmooth, double omega, int64_t coarse_sweeps,
+                          double* norm, void* stream);
+
 /* ---------------------------------------------------------------- tuning
  * Launch-shape knobs (process-wide; take effect for the next launches and for
  * graphs recorded afterwards).  Keys: "sweep.kernel" (1: workgroup tiles

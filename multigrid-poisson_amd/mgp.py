@@ -114,6 +114,7 @@ _SIGS = {
     "mgp_fill_f16": [P(Array), d, vp], "mgp_rbgs_half_f16": [P(Array), P(Array), BOX, d, d, i32, vp],
     "mgp_rbgs_sweeps_f16": [P(Array), P(Array), BOX, d, d, i64, vp],
     "mgp_rbgs_sweeps_f32_f64": [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
+    "mgp_cycle_noalloc_f64": [ctypes.c_char_p, P(Array), P(Array), P(Faces), i64, i64, i64, d, i64, vp, vp],
     "mgp_residual_f16": [P(Array), P(Array), P(Array), BOX, d, vp, vp],
     "mgp_restrict_fw_scaled_f16": [P(Array), P(Array), vp, vp],
     "mgp_restrict_fw_scaled_f32_f16": [P(Array), P(Array), vp, vp],
