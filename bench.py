@@ -72,6 +72,8 @@ def main():
     ap.add_argument("--levels", type=int, default=7)
     ap.add_argument("--tol", type=float, default=1e-10)
     ap.add_argument("--nsmooth", type=int, default=2)
+    ap.add_argument("--omega", type=float, default=1.0,
+                    help="SOR factor of the red-black Gauss-Seidel smoother in the headline and mixed legs")
     ap.add_argument("--coarse-sweeps", type=int, default=32,
                     help="smoothing passes per visit of the coarsest level (17x17x9)")
     ap.add_argument("--kernel-reps", type=int, default=10)
@@ -93,6 +95,9 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run once each: a fill (write-only) and an inf-norm (read-only) of one 4.3 GB array, "
                          "then one fine-level one-pass sweep of the workload; no JSON line (for rocprofv3 --pmc)")
+    ap.add_argument("--probe-mixed", type=int, default=0, metavar="PROBLEM",
+                    help="two FP32-refinement solves of problem 2 or 3 launch by launch (for kernel traces); "
+                         "no JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build runs the solve on one GPU; --gpus %d is not supported "
@@ -102,6 +107,9 @@ def main():
     mgp.require_gpu()
     if args.traffic_probe:
         traffic_probe(args)
+        return
+    if args.probe_mixed:
+        probe_mixed(args, args.probe_mixed)
         return
     for kv in args.tune:
         k, v = kv.split("=")
@@ -196,7 +204,7 @@ def main():
         "dtype": "f64",
         "data": "synthetic (analytic u = x^3 y^2 z)",
         "config": {"workload": "poisson-fcycle-%dx%dx%d" % dims, "grid": list(dims), "levels": args.levels,
-                   "cycle": "F", "smoother": "red-black GS omega=1, %d sweeps" % args.nsmooth,
+                   "cycle": "F", "smoother": "red-black GS SOR omega=%g, %d sweeps" % (args.omega, args.nsmooth),
                    "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
                    "coarsest_sweeps": args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
@@ -233,13 +241,13 @@ def timed_solves(s, args):
     cycles per solve, final relative residuals)."""
     for _ in range(args.warmup):
         s.reset()
-        s.solve("Fcycle", args.nsmooth, 1.0)
+        s.solve("Fcycle", args.nsmooth, args.omega)
     s.sync()
     cycles, rel = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.reset()
-        it, rr, _t = s.solve("Fcycle", args.nsmooth, 1.0)
+        it, rr, _t = s.solve("Fcycle", args.nsmooth, args.omega)
         cycles.append(it + 1)
         rel.append(rr)
     s.sync()
@@ -321,9 +329,16 @@ def mixed_leg(dims, h, args):
 
 
 def traffic_probe(args):
-    """Kernels for HBM-counter runs: fill of r (8 B/point written), inf-norm of
-    f (8 B/point read), one one-pass sweep u -> r (24 B per box point) and one
-    two-sweep pass u -> r (24 B per box point)."""
+    """Kernels for HBM-counter runs, each launched once on the headline grid:
+    calibration kernels of known size -- a fill of r (8 B/point written), an
+    inf-norm of f (8 B/point read), r += f (mgp_add_f64: 16 B/point read,
+    8 B/point written) -- then the fine-level operators of the cycle: the
+    one-pass sweep u -> r and the two-sweep pass u -> r (24 B per box point),
+    the residual with its norm (24 B), the norm-only residual (16 B), the fused
+    residual + full weighting into level 1's f (16 B per fine point read,
+    1 B written), the prolongation + correction from level 1 (16 B per fine
+    point, 1 B read), and the hi + lo update + residual of mixed refinement
+    (40 B per point: u, f 8 B each, lo, e 4 B each read; u, lo, rhs written)."""
     import numpy as np
 
     n, nz = args.n, args.n // 2 + 1
@@ -333,6 +348,7 @@ def traffic_probe(args):
     s.to_device()
     L = mgp.lib()
     u, f_, r = s.device_array("u"), s.device_array("f"), s.device_array("r")
+    u1, f1 = s.device_array("u", 1), s.device_array("f", 1)
     scal = mgp.DeviceBuffer(np.zeros(1))
     F = mgp.Faces()
     for q in range(6):
@@ -343,17 +359,53 @@ def traffic_probe(args):
         F.g[q] = a.data if t == mgp.NEUMANN else None
     F.h = h
     L.mgp_solver_sync(s.h)
+    shape, halo = [u.shape[0], u.shape[1], u.shape[2]], [u.halo[q] for q in range(6)]
+    lo, ln, e, out = (mgp.DeviceArray(shape, halo, np.float32) for _ in range(4))
+    un = mgp.DeviceArray(shape, halo)
+    for a in (lo, ln, e, out):
+        mgp.check(L.mgp_fill_f32(a.ref(), 0.0, None))
     box = mgp.box([0, n, 0, n, 1, nz - 1])
-    mgp.check(L.mgp_fill_f64(ctypes.byref(r), 0.0, None))
-    mgp.check(L.mgp_norm_inf_f64(ctypes.byref(f_), scal.ptr, None))
-    mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_), ctypes.byref(F),
-                                   box, 1.0, None))
-    mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r), ctypes.byref(f_),
-                                    ctypes.byref(F), box, 1.0, None))
+    R = ctypes.byref
+    mgp.check(L.mgp_fill_f64(R(r), 0.0, None))
+    mgp.check(L.mgp_norm_inf_f64(R(f_), scal.ptr, None))
+    mgp.check(L.mgp_add_f64(R(r), R(f_), None))
+    mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, R(u), R(r), R(f_), R(F), box, 1.0, None))
+    mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, R(u), R(r), R(f_), R(F), box, 1.0, None))
+    mgp.check(L.mgp_residual_f64(mgp.CONSISTENT, R(u), R(f_), R(r), box, h, scal.ptr, None))
+    mgp.check(L.mgp_residual_norm_f64(mgp.CONSISTENT, R(u), R(f_), R(r), box, h, scal.ptr, None))
+    mgp.check(L.mgp_residual_restrict_fw_f64(mgp.CONSISTENT, R(u), R(f_), R(r), R(f1), box, h, None))
+    mgp.check(L.mgp_prolong_add_f64(R(u1), R(u), None))
+    mgp.check(L.mgp_update_residual_hilo_f64_f32(R(u), un.ref(), lo.ref(), ln.ref(), e.ref(), R(f_), R(r), out.ref(),
+                                                 R(F), box, scal.ptr, None))
     L.mgp_stream_sync(None)
     pts = n * n * nz
-    log("traffic probe: fill %d B, norm %d B, sweep %d B, two-sweep pass %d B algorithmic" %
-        (8 * pts, 8 * pts, 24 * n * n * (nz - 2), 24 * n * n * (nz - 2)))
+    bpts = n * n * (nz - 2)
+    log("traffic probe (algorithmic bytes): fill %d, norm %d, add %d, sweep %d, two-sweep pass %d, residual %d, "
+        "norm-only residual %d, residual + FW %d, prolongation + add %d, hi+lo update + residual %d" %
+        (8 * pts, 8 * pts, 24 * pts, 24 * bpts, 24 * bpts, 24 * bpts, 16 * bpts, 16 * pts + 8 * pts // 8,
+         16 * pts + 8 * pts // 8, 40 * pts))
+    s.close()
+
+
+def probe_mixed(args, problem):
+    """One warm-up and one measured FP32-refinement solve (launch by launch,
+    for kernel traces) of the mixed leg's problem `problem` on the headline
+    grid: 2 = u = x^3 y^2 z with Neumann x/y faces, 3 = cos(x z^2) sin(y^3)
+    with Dirichlet faces; no JSON line."""
+    n, nz = args.n, args.n // 2 + 1
+    h = 1.0 / (n - 1)
+    bc = "NNNNDD" if problem == 2 else "DDDDDD"
+    s = mgp.Solver((n, n, nz), args.levels, h, (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                   mgp.FULL_WEIGHTING, problem=problem, dtype=mgp.MIXED, flags=mgp.QUIET | mgp.NO_GRAPHS,
+                   maxiter=30, miniter=1, tolerance=args.tol, coarse_sweeps=args.coarse_sweeps)
+    s.to_device()
+    for k in range(2):
+        s.reset()
+        s.sync()
+        t0 = time.perf_counter()
+        it, rel, _ = s.solve("Fcycle", args.nsmooth, 1.0)
+        log("probe mixed p%d solve %d: %.1f ms, %d iterations, rel_res %.3e" %
+            (problem, k, 1e3 * (time.perf_counter() - t0), it, rel))
     s.close()
 
 
@@ -457,34 +509,40 @@ def serve_leg(args):
     sample of the same requests (fresh hierarchy each, as the server's
     requests of new shapes get): the burst capacity over all requests, then
     Poisson arrivals offered at args.serve_load x the one-at-a-time rate,
-    with the delivered rate and p50 / p99 latency over all of them."""
+    with the delivered rate and p50 / p99 latency over all of them.  Both
+    solve the coarsest level by CG (the server always; the one-at-a-time
+    solver with MGP_SOLVER_COARSE_CG)."""
     import numpy as np
 
     rng = np.random.default_rng(args.seed + 5)
     reqs = serve_requests(rng, args.serve_count)
     tol = 1e-8
     L = mgp.lib()
-    # one-at-a-time FP64 on a sample
+    # one-at-a-time FP64 on a sample, with and without cycle graphs (the
+    # faster counts)
     sample = reqs[:min(len(reqs), args.serve_sample)]
     log("bench: serving leg, %d requests; one-at-a-time FP64 on %d" % (len(reqs), len(sample)))
-    t0 = time.perf_counter()
-    for d, bc, modes in sample:
-        s = mgp.Solver(d, max_levels(d), 1.0 / (d[0] - 1), (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
-                       mgp.FULL_WEIGHTING, problem=None, dtype=mgp.F64, maxiter=100, miniter=1, tolerance=tol,
-                       coarse_sweeps=32)
-        mgp.check(L.mgp_solver_to_device_zero(s.h))
-        fa = s.device_array("f")
-        abc = (ctypes.c_int32 * 48)(*[v for a, b_, c, _ in modes for v in (a, b_, c)])
-        amp = (ctypes.c_double * 16)(*[m[3] for m in modes])
-        mgp.check(L.mgp_fill_modes_f64(ctypes.byref(fa), len(modes), abc, amp, 1.0 / (d[0] - 1), None))
-        it, rr, _ = s.solve("Fcycle", 2, 1.0)
-        ua = s.device_array("u")
-        host = np.empty(int(np.prod(mgp.padded_shape(list(ua.shape), list(ua.halo)))))
-        mgp.check(L.mgp_memcpy_d2h(host.ctypes.data, ua.data, host.nbytes, None))
-        s.close()
-    seq_s = time.perf_counter() - t0
-    seq_rate = len(sample) / seq_s
-    log("bench:   %.2f s, %.1f solves/s" % (seq_s, seq_rate))
+    seq = {}
+    for name, gflag in (("graphs", 0), ("no_graphs", mgp.NO_GRAPHS)):
+        t0 = time.perf_counter()
+        for d, bc, modes in sample:
+            s = mgp.Solver(d, max_levels(d), 1.0 / (d[0] - 1), (0.0, 0.0, 0.0), bc, mgp.CONSISTENT,
+                           mgp.GAUSS_SEIDEL, mgp.FULL_WEIGHTING, problem=None, dtype=mgp.F64,
+                           flags=mgp.QUIET | mgp.COARSE_CG | gflag, maxiter=100, miniter=1, tolerance=tol,
+                           coarse_sweeps=32)
+            mgp.check(L.mgp_solver_to_device_zero(s.h))
+            fa = s.device_array("f")
+            abc = (ctypes.c_int32 * 48)(*[v for a, b_, c, _ in modes for v in (a, b_, c)])
+            amp = (ctypes.c_double * 16)(*[m[3] for m in modes])
+            mgp.check(L.mgp_fill_modes_f64(ctypes.byref(fa), len(modes), abc, amp, 1.0 / (d[0] - 1), None))
+            it, rr, _ = s.solve("Fcycle", 2, 1.0)
+            ua = s.device_array("u")
+            host = np.empty(int(np.prod(mgp.padded_shape(list(ua.shape), list(ua.halo)))))
+            mgp.check(L.mgp_memcpy_d2h(host.ctypes.data, ua.data, host.nbytes, None))
+            s.close()
+        seq[name] = len(sample) / (time.perf_counter() - t0)
+        log("bench:   one at a time (%s): %.1f solves/s" % (name, seq[name]))
+    seq_rate = max(seq.values())
 
     def run(arrival_rate):
         srv = mgp.Server(hbm_budget=96 << 30, max_active=args.serve_workers, tolerance=tol, coarse_sweeps=32)
@@ -526,6 +584,7 @@ def serve_leg(args):
     return {"workload": "serving-%d-requests-2^k+1-mixed-bc" % len(reqs), "tolerance": tol,
             "workers": args.serve_workers,
             "one_at_a_time_fp64_solves/s": seq_rate,
+            "one_at_a_time_fp64_by_launch_mode": seq,
             "one_at_a_time_sample": "%d of %d requests" % (len(sample), len(reqs)),
             "burst": burst, "poisson": poisson,
             "capacity_vs_fp64_one_at_a_time": burst["solves/s"] / seq_rate,

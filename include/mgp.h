@@ -223,6 +223,23 @@ int mgp_rbgs_sweeps_f32(int mode, const mgp_array* u_in, const mgp_array* u_out,
  * 8-byte elements. */
 int mgp_rbgs_sweeps_f32_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                             const mgp_faces* faces, const int64_t box[6], double omega, int64_t nsweeps, void* stream);
+/* The coarsest level solved by conjugate gradients in FP64 in one workgroup
+ * (consistent mode; no counterpart in the reference, whose coarsest level
+ * gets the smoothing passes of the others, include/vcycle.h:27-31): with
+ * res = f - L u (Neumann ghosts sign*2h*g + mirror from `faces`, Dirichlet
+ * boundary nodes as stored), CG on the symmetric form -W L0 d = -W res of the
+ * homogeneous operator L0 (W = 1/2 per Neumann boundary coordinate) from d = 0
+ * until |r| <= rtol |r0| or max_iters steps, then u += d in place.  The box
+ * must be the level's natural box (boundary nodes of Dirichlet faces out,
+ * those of Neumann faces in) and the level small enough for one workgroup:
+ * (nx+2)(ny+2)(nz+2) doubles in 64 KiB LDS, at most 8192 box points and 4096
+ * points per face (MGP_ERR_ARG otherwise).  At least one face Dirichlet
+ * (else the system is singular).  Fixed-order reductions: the same bits on
+ * every run. */
+int mgp_cg_solve_f64(const mgp_array* u, const mgp_array* f, const mgp_faces* faces, const int64_t box[6],
+                     int64_t max_iters, double rtol, void* stream);
+int mgp_cg_solve_f32(const mgp_array* u, const mgp_array* f, const mgp_faces* faces, const int64_t box[6],
+                     int64_t max_iters, double rtol, void* stream);
 /* Weighted Jacobi over the box: u = (1-w)v + w/6 (sum_nb v - h^2 f)
  * (include/jacobi.h:39-82).  v's halos must hold current ghost values. */
 int mgp_jacobi_f64(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t box[6],
@@ -304,6 +321,10 @@ enum { MGP_JACOBI = 0, MGP_GAUSS_SEIDEL = 1 };
 /* MGP_MIXED: the FP32 cycle's coarsest level smoothed in FP32 arithmetic
  * instead of FP64 (Settings::refine_coarse_fp64 = false) */
 #define MGP_SOLVER_COARSE_FP32 16
+/* consistent mode: every visit of the coarsest level solves it by CG in FP64
+ * (mgp_cg_solve_*, to 1e-8 relative) instead of coarse_sweeps smoothing passes,
+ * where the level fits one workgroup (Settings::coarse_cg) */
+#define MGP_SOLVER_COARSE_CG 32
 
 typedef struct mgp_solver_config {
     int64_t dims[3];     /* Settings::dims (include/settings.h:17) */
@@ -470,10 +491,16 @@ typedef struct {
     int32_t relaxation;     /* MGP_GAUSS_SEIDEL or MGP_JACOBI */
     int64_t nsmooth;        /* smoothing passes (pre = post) */
     double omega;           /* <= 0: the smoother's default */
-    int64_t coarse_sweeps;  /* passes on the coarsest level; 0: nsmooth */
+    int64_t coarse_sweeps;  /* passes on a coarsest level too large for the CG solve
+                               (mgp_cg_solve_*, used wherever the level fits); 0: nsmooth */
     int64_t maxiter;        /* refinement iterations at most */
     double tolerance;       /* on ||r||inf/||f||inf of the FP64 residual */
+    int32_t flags;          /* MGP_SERVER_* */
 } mgp_server_config;
+/* record each hierarchy's cycles as HIP graphs (replayed by later iterations
+ * and requests of the same shape); without it every kernel is launched
+ * directly, which is cheaper for hierarchies used for a few iterations only */
+#define MGP_SERVER_GRAPHS 1
 typedef struct {
     int64_t dims[3];                   /* 2^k m + 1 per axis; h = 1/(dims[0]-1) */
     int32_t bc[6];                     /* MGP_DIRICHLET / MGP_NEUMANN, order N S E W T B */

@@ -313,6 +313,178 @@ __global__ void __launch_bounds__(1024) k_rbgs_lds(View<S> ui, View<S> uo, View<
     }
 }
 
+// ------------------------------------------------- coarsest-level CG solve
+// The coarsest level of a cycle solved by conjugate gradients in FP64 in one
+// workgroup (mgp_cg_solve_*; no counterpart in the reference, whose coarsest
+// level gets the same few smoothing passes as the others, include/vcycle.h:
+// 27-31).  Smoothing passes leave the smooth error of a coarsest level with a
+// long axis and few Dirichlet faces nearly untouched (a 33-point axis with one
+// Dirichlet end needs thousands of Gauss-Seidel passes), which stalls the
+// cycle; CG reduces it in O(sqrt(condition)) steps.
+//
+// With res = f - L u (Neumann ghosts sign*2h*g + mirror, Dirichlet boundary
+// nodes fixed), the correction d solves L0 d = res, L0 the operator with
+// homogeneous data.  L0 is not symmetric at Neumann boundary nodes (the ghost
+// doubles the inward coupling), but W L0 is, with W = 1/2 per Neumann
+// boundary coordinate of a point; CG runs on the SPD system -W L0 d = -W res
+// from d = 0 until |r| <= rtol |r0| or max_iters, then u += d.  The search
+// direction p lives in LDS over the shape padded by one layer (ghost layer:
+// the mirror value, Dirichlet nodes: 0); each thread keeps the d, r and
+// weights of its points in registers.  Reductions are fixed-order trees, so
+// the result does not depend on timing.
+__device__ __forceinline__ double block_sum(double v, double* part) {
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = (blockDim.x + 63) >> 6;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if (lane == 0) part[wave] = v;
+    __syncthreads();
+    double t = 0.0;
+    for (int w = 0; w < nwaves; w++) t += part[w];
+    __syncthreads();  // part is reused by the next reduction
+    return t;
+}
+
+template <class S>
+__global__ void __launch_bounds__(1024) k_cg_lds(View<S> u, View<S> f, Faces<S> F, Box bx, int64_t max_iters,
+                                                  double rtol, double two_h, double inv_h2) {
+    constexpr int MAXP = 8;  // box points per thread (<= 8192 / 1024)
+    extern __shared__ unsigned char lds_raw[];
+    __shared__ double part[16];
+    double* sp = reinterpret_cast<double*>(lds_raw);
+    const int n0 = (int)u.n0, n1 = (int)u.n1, n2 = (int)u.n2;
+    const int q2 = n2 + 2, q1 = n1 + 2;
+    const int P = (n0 + 2) * q1 * q2;
+    auto L = [&](int i, int j, int k) { return ((i + 1) * q1 + (j + 1)) * q2 + (k + 1); };
+    const int tid = threadIdx.x, nt = blockDim.x;
+    for (int t = tid; t < P; t += nt) sp[t] = 0.0;
+    const int bn0 = (int)(bx.x1 - bx.x0), bn1 = (int)(bx.y1 - bx.y0), bn2 = (int)(bx.z1 - bx.z0);
+    const int nbox = bn0 * bn1 * bn2;
+    int pc[MAXP];
+    double w[MAXP], d[MAXP], r[MAXP];
+    double rr = 0.0;
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        const int t = tid + m * nt;
+        pc[m] = -1;
+        w[m] = d[m] = r[m] = 0.0;
+        if (t >= nbox) continue;
+        const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
+        pc[m] = L(i, j, k);
+        // weight: 1/2 per coordinate on a Neumann face
+        double wt = 1.0;
+        if ((i == 0 && F.neumann[MGP_WEST]) || (i == n0 - 1 && F.neumann[MGP_EAST])) wt *= 0.5;
+        if ((j == 0 && F.neumann[MGP_SOUTH]) || (j == n1 - 1 && F.neumann[MGP_NORTH])) wt *= 0.5;
+        if ((k == 0 && F.neumann[MGP_BOTTOM]) || (k == n2 - 1 && F.neumann[MGP_TOP])) wt *= 0.5;
+        w[m] = wt;
+        // res = f - L u with the stored boundary values and Neumann ghosts
+        const double uc = (double)u.at(i, j, k);
+        double sum = 0.0;
+#pragma unroll
+        for (int ax = 0; ax < 3; ax++)
+#pragma unroll
+            for (int dir = -1; dir <= 1; dir += 2) {
+                int a[3] = {i, j, k};
+                const int n[3] = {n0, n1, n2};
+                a[ax] += dir;
+                double nb;
+                if (a[ax] >= 0 && a[ax] < n[ax]) {
+                    nb = (double)u.at(a[0], a[1], a[2]);
+                } else {
+                    const int face = ax == 0 ? (dir > 0 ? MGP_EAST : MGP_WEST)
+                                             : ax == 1 ? (dir > 0 ? MGP_NORTH : MGP_SOUTH) : (dir > 0 ? MGP_TOP : MGP_BOTTOM);
+                    a[ax] -= 2 * dir;
+                    const int gi = ax == 0 ? j * n2 + k : ax == 1 ? i * n2 + k : i * n1 + j;
+                    const double g = F.g[face] ? (double)F.g[face][gi] : 0.0;
+                    nb = (double)dir * two_h * g + (double)u.at(a[0], a[1], a[2]);
+                }
+                sum += nb - uc;
+            }
+        const double res = fma(-inv_h2, sum, (double)f.at(i, j, k));
+        r[m] = -wt * res;  // c = -W res; r0 = c with d = 0
+        rr += r[m] * r[m];
+    }
+    __syncthreads();  // sp zeroed before p is written
+#pragma unroll
+    for (int m = 0; m < MAXP; m++)
+        if (pc[m] >= 0) sp[pc[m]] = r[m];
+    rr = block_sum(rr, part);
+    const double stop = rtol * rtol * rr;
+    // ghost cells of the padded shape on Neumann faces: p(ghost) = p(mirror);
+    // each thread owns up to GMAX cells per face (faces of <= GMAX * 1024
+    // cells; a level that fits the LDS has faces of at most ~2700)
+    constexpr int GMAX = 4;
+    const int s0l = q1 * q2, s1l = q2;
+    int gc[6][GMAX], gd[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+        gd[q] = q == MGP_EAST ? -2 * s0l : q == MGP_WEST ? 2 * s0l : q == MGP_NORTH ? -2 * s1l
+              : q == MGP_SOUTH ? 2 * s1l : q == MGP_TOP ? -2 : 2;
+        const int a0 = (q == MGP_EAST || q == MGP_WEST) ? 1 : n0;
+        const int a1 = (q == MGP_NORTH || q == MGP_SOUTH) ? 1 : n1;
+        const int a2 = (q == MGP_TOP || q == MGP_BOTTOM) ? 1 : n2;
+#pragma unroll
+        for (int g = 0; g < GMAX; g++) {
+            const int t = tid + g * nt;
+            gc[q][g] = -1;
+            if (!F.neumann[q] || t >= a0 * a1 * a2) continue;
+            const int i = t / (a1 * a2), j = (t / a2) % a1, k = t % a2;
+            switch (q) {
+            case MGP_EAST: gc[q][g] = L(n0, j, k); break;
+            case MGP_WEST: gc[q][g] = L(-1, j, k); break;
+            case MGP_NORTH: gc[q][g] = L(i, n1, k); break;
+            case MGP_SOUTH: gc[q][g] = L(i, -1, k); break;
+            case MGP_TOP: gc[q][g] = L(i, j, n2); break;
+            default: gc[q][g] = L(i, j, -1); break;
+            }
+        }
+    }
+    for (int64_t it = 0; it < max_iters && rr > stop && rr > 0.0; it++) {
+#pragma unroll
+        for (int q = 0; q < 6; q++)
+#pragma unroll
+            for (int g = 0; g < GMAX; g++)
+                if (gc[q][g] >= 0) sp[gc[q][g]] = sp[gc[q][g] + gd[q]];
+        __syncthreads();
+        double qv[MAXP], pq = 0.0;
+#pragma unroll
+        for (int m = 0; m < MAXP; m++) {
+            qv[m] = 0.0;
+            if (pc[m] < 0) continue;
+            const int c = pc[m];
+            const double pcv = sp[c];
+            const double sum = ((sp[c - s0l] - pcv) + (sp[c + s0l] - pcv)) + ((sp[c - s1l] - pcv) + (sp[c + s1l] - pcv)) +
+                               ((sp[c - 1] - pcv) + (sp[c + 1] - pcv));
+            qv[m] = -w[m] * inv_h2 * sum;  // (-W L0 p)
+            pq += pcv * qv[m];
+        }
+        pq = block_sum(pq, part);
+        if (!(pq > 0.0)) break;
+        const double alpha = rr / pq;
+        double rn = 0.0;
+#pragma unroll
+        for (int m = 0; m < MAXP; m++) {
+            if (pc[m] < 0) continue;
+            d[m] += alpha * sp[pc[m]];
+            r[m] -= alpha * qv[m];
+            rn += r[m] * r[m];
+        }
+        rn = block_sum(rn, part);
+        const double beta = rn / rr;
+        rr = rn;
+#pragma unroll
+        for (int m = 0; m < MAXP; m++)
+            if (pc[m] >= 0) sp[pc[m]] = r[m] + beta * sp[pc[m]];
+        __syncthreads();
+    }
+#pragma unroll
+    for (int m = 0; m < MAXP; m++) {
+        if (pc[m] < 0) continue;
+        const int t = tid + m * nt;
+        const int i = (int)bx.x0 + t / (bn1 * bn2), j = (int)bx.y0 + (t / bn2) % bn1, k = (int)bx.z0 + t % bn2;
+        u.at(i, j, k) = (S)((double)u.at(i, j, k) + d[m]);
+    }
+}
+
 // include/jacobi.h:39-82: u = J(v), out of place; the reference writes
 // u = (1-w)v and then adds, which is the same two roundings
 template <class T, int MODE>
@@ -847,6 +1019,43 @@ static int rbgs_sweeps(int mode, const mgp_array* ui, const mgp_array* uo, const
     return launch_status();
 }
 
+// the coarsest-level CG solve (consistent mode; see k_cg_lds)
+template <class S>
+static int cg_solve(const mgp_array* u, const mgp_array* f, const mgp_faces* faces, const int64_t b[6],
+                    int64_t max_iters, double rtol, void* stream) {
+    if (!valid_array(u) || !valid_array(f) || !faces || !same_shape(u, f) || max_iters < 0 || !(rtol >= 0.0) ||
+        !(faces->h > 0.0))
+        return MGP_ERR_ARG;
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < 0 || b[2 * d + 1] > u->shape[d]) return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == u->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == u->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == u->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    // a Neumann face must be the box's face (its points are unknowns)
+    if ((faces->type[MGP_WEST] == MGP_NEUMANN && bx.x0 != 0) ||
+        (faces->type[MGP_EAST] == MGP_NEUMANN && bx.x1 != u->shape[0]) ||
+        (faces->type[MGP_SOUTH] == MGP_NEUMANN && bx.y0 != 0) ||
+        (faces->type[MGP_NORTH] == MGP_NEUMANN && bx.y1 != u->shape[1]) ||
+        (faces->type[MGP_BOTTOM] == MGP_NEUMANN && bx.z0 != 0) ||
+        (faces->type[MGP_TOP] == MGP_NEUMANN && bx.z1 != u->shape[2]))
+        return MGP_ERR_ARG;
+    const int64_t n0 = u->shape[0], n1 = u->shape[1], n2 = u->shape[2];
+    const size_t lds = (size_t)(n0 + 2) * (n1 + 2) * (n2 + 2) * sizeof(double);
+    if (lds > 65536 - 256) return MGP_ERR_ARG;  // with the kernel's 128-byte reduction array
+    if ((b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4]) > 8 * 1024) return MGP_ERR_ARG;
+    if (n1 * n2 > 4096 || n0 * n2 > 4096 || n0 * n1 > 4096) return MGP_ERR_ARG;  // 4 ghost cells per thread
+    if (box_empty(bx) || max_iters == 0) return MGP_OK;
+    const Faces<S> F = faces_of<S>(*faces);
+    hipLaunchKernelGGL(k_cg_lds<S>, dim3(1), dim3(1024), lds, (hipStream_t)stream, view_of<S>(*u), view_of<S>(*f), F,
+                       bx, max_iters, rtol, 2.0 * faces->h, 1.0 / (faces->h * faces->h));
+    return launch_status();
+}
+
 template <class T>
 static int jacobi(int mode, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6],
                   double omega, double h, void* stream) {
@@ -1308,6 +1517,14 @@ int mgp_rbgs_sweeps_f32(int m, const mgp_array* ui, const mgp_array* uo, const m
 int mgp_rbgs_sweeps_f32_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                             const int64_t b[6], double w, int64_t n, void* s) {
     return rbgs_sweeps<float, double>(m, ui, uo, f, F, b, w, n, s);
+}
+int mgp_cg_solve_f64(const mgp_array* u, const mgp_array* f, const mgp_faces* F, const int64_t b[6], int64_t max_iters,
+                     double rtol, void* s) {
+    return cg_solve<double>(u, f, F, b, max_iters, rtol, s);
+}
+int mgp_cg_solve_f32(const mgp_array* u, const mgp_array* f, const mgp_faces* F, const int64_t b[6], int64_t max_iters,
+                     double rtol, void* s) {
+    return cg_solve<float>(u, f, F, b, max_iters, rtol, s);
 }
 int mgp_jacobi_f64(int m, const mgp_array* u, const mgp_array* v, const mgp_array* f, const int64_t b[6], double w,
                    double h, void* s) {

@@ -21,6 +21,7 @@
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <list>
@@ -105,6 +106,14 @@ struct Slot : SlotBase {
         st.tolerance = c.tolerance;
         st.coarse_sweeps = c.coarse_sweeps;
         st.mode = Poisson::CONSISTENT;
+        // the coarsest level solved by CG (mgp_cg_solve_*): with 2^k+1 axes
+        // drawn independently the coarsest level can keep a 33-point axis,
+        // whose smooth error smoothing passes barely reduce
+        st.coarse_cg = true;
+        // a new hierarchy (a shape not seen recently) serves a few iterations:
+        // recording and instantiating its cycle graphs costs more than the
+        // launches they save, unless the caller asks for graphs
+        st.use_graphs = (c.flags & MGP_SERVER_GRAPHS) != 0;
         st.print_cycle_time = false;
         st.print_settings = false;
         auto t = [&](int q) { return r.bc[q] == MGP_NEUMANN ? Poisson::NEUMANN : Poisson::DIRICHLET; };
@@ -224,11 +233,12 @@ private:
         for (int q = 0; q < 6; q++) bits |= (uint32_t)(r.bc[q] == MGP_NEUMANN) << q;
         return Key{r.dims[0], r.dims[1], r.dims[2], bits};
     }
-    // device bytes of a request's hierarchy: level 0 in FP64 (u, u_alt, f, r)
-    // and the FP32 hierarchy (u, u_alt, f, r; 8/7 of level 0), halos included
+    // device bytes of a request's hierarchy: level 0 in FP64 (u, u_alt, f, r),
+    // the two FP32 lo words of its hi + lo iterate and the FP32 hierarchy (u,
+    // u_alt, f, r; 8/7 of level 0), halos included
     static size_t need_of(const mgp_request& r) {
         const double pts = (double)(r.dims[0] + 2) * (double)(r.dims[1] + 2) * (double)(r.dims[2] + 2);
-        return (size_t)(pts * (4 * 8 + 4 * 4 * 8.0 / 7.0) * 1.05) + (4u << 20);
+        return (size_t)(pts * (4 * 8 + 2 * 4 + 4 * 4 * 8.0 / 7.0) * 1.05) + (4u << 20);
     }
 
     // under mu: the oldest request and a resident slot for it, or a
@@ -302,6 +312,10 @@ private:
                 s->stats(&j->res.iterations, &j->res.rel_res);
             } catch (const std::exception& e) {
                 mgp::set_last_error(e.what());
+                // the caller sees only the status: the reason goes to stderr
+                std::fprintf(stderr, "mgp server: request %lld (%lldx%lldx%lld) failed: %s\n", (long long)j->id,
+                             (long long)j->req.dims[0], (long long)j->req.dims[1], (long long)j->req.dims[2],
+                             e.what());
                 status = MGP_ERR_STATE;
             }
             // a slot whose request failed may hold a broken stream or graph:

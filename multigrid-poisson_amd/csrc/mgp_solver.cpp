@@ -210,6 +210,9 @@ int mgp_solver_create(const mgp_solver_config* c, void** out) {
         st.print_settings = !(c->flags & MGP_SOLVER_QUIET);
         st.refine_hilo = !(c->flags & MGP_SOLVER_PLAIN_ITERATE);
         st.refine_coarse_fp64 = !(c->flags & MGP_SOLVER_COARSE_FP32);
+        st.coarse_cg = (c->flags & MGP_SOLVER_COARSE_CG) != 0;
+        if (st.coarse_cg && c->mode != MGP_MODE_CONSISTENT)
+            throw std::invalid_argument("MGP_SOLVER_COARSE_CG needs MGP_MODE_CONSISTENT");
         if (c->refine_polish < 0) throw std::invalid_argument("refine_polish must be >= 0");
         st.refine_polish = c->refine_polish;
         if (c->half_from < 0) throw std::invalid_argument("half_from must be >= 0");

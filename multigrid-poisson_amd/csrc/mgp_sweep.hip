@@ -460,6 +460,14 @@ __device__ __forceinline__ T lane_next(T v) {  // value of lane l+1 (z + 1)
 // check does not cover the SGPR offset.  Loads always use in-range offsets.
 typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
 constexpr uint32_t BUF_OOB = 0x40000000u;  // offset past any plane (planes < 1 GiB)
+// cache-policy bits of the buffer loads and stores (gfx950: bit 0 sc0, bit 1
+// nt, bit 4 sc1); a build-time choice, measured per kernel
+#ifndef MGP_LOAD_POLICY
+#define MGP_LOAD_POLICY 0
+#endif
+#ifndef MGP_STORE_POLICY
+#define MGP_STORE_POLICY 0
+#endif
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t plane_rsrc(const void* base, uint32_t bytes) {
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
@@ -468,17 +476,18 @@ template <class T>
 __device__ __forceinline__ T bload(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff);
 template <>
 __device__ __forceinline__ double bload<double>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, MGP_LOAD_POLICY));
 }
 template <>
 __device__ __forceinline__ float bload<float>(__amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, MGP_LOAD_POLICY));
 }
 __device__ __forceinline__ void bstore(double v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, v), r, (int)voff, (int)soff, MGP_STORE_POLICY);
 }
 __device__ __forceinline__ void bstore(float v, __amdgpu_buffer_rsrc_t r, uint32_t voff, uint32_t soff) {
-    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned int, v), r, (int)voff, (int)soff,
+                                          MGP_STORE_POLICY);
 }
 
 // EDGE: the tile has rows outside the box or in-plane Neumann ghost positions

@@ -10,7 +10,9 @@ import re
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "lib", "libmgp.so")
+# MGP_LIBRARY: another build of the library (same C ABI), e.g. with other
+# build-time cache policies, for measurements
+LIB_PATH = os.environ.get("MGP_LIBRARY") or os.path.join(HERE, "lib", "libmgp.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "mgp.h")
 
 NORTH, SOUTH, EAST, WEST, TOP, BOTTOM = range(6)
@@ -19,7 +21,7 @@ REFERENCE, CONSISTENT = 0, 1
 INJECTION, FULL_WEIGHTING = 0, 1
 JACOBI, GAUSS_SEIDEL = 0, 1
 F64, F32, MIXED = 0, 1, 2
-NO_GRAPHS, NO_FUSED, QUIET, PLAIN_ITERATE, COARSE_FP32 = 1, 2, 4, 8, 16
+NO_GRAPHS, NO_FUSED, QUIET, PLAIN_ITERATE, COARSE_FP32, COARSE_CG = 1, 2, 4, 8, 16, 32
 OK, ERR_ARG, ERR_NODEV, ERR_STATE = 0, -1, -2, -3
 LOC = "NSEWTB"
 
@@ -60,7 +62,10 @@ MAX_MODES = 16
 class ServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_int64), ("max_active", ctypes.c_int32), ("relaxation", ctypes.c_int32),
                 ("nsmooth", ctypes.c_int64), ("omega", ctypes.c_double), ("coarse_sweeps", ctypes.c_int64),
-                ("maxiter", ctypes.c_int64), ("tolerance", ctypes.c_double)]
+                ("maxiter", ctypes.c_int64), ("tolerance", ctypes.c_double), ("flags", ctypes.c_int32)]
+
+
+SERVER_GRAPHS = 1
 
 
 class Request(ctypes.Structure):
@@ -114,6 +119,8 @@ _SIGS = {
     "mgp_fill_f16": [P(Array), d, vp], "mgp_rbgs_half_f16": [P(Array), P(Array), BOX, d, d, i32, vp],
     "mgp_rbgs_sweeps_f16": [P(Array), P(Array), BOX, d, d, i64, vp],
     "mgp_rbgs_sweeps_f32_f64": [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
+    "mgp_cg_solve_f64": [P(Array), P(Array), P(Faces), BOX, i64, d, vp],
+    "mgp_cg_solve_f32": [P(Array), P(Array), P(Faces), BOX, i64, d, vp],
     "mgp_cycle_noalloc_f64": [ctypes.c_char_p, P(Array), P(Array), P(Faces), i64, i64, i64, d, i64, vp, vp],
     "mgp_residual_f16": [P(Array), P(Array), P(Array), BOX, d, vp, vp],
     "mgp_restrict_fw_scaled_f16": [P(Array), P(Array), vp, vp],
@@ -473,12 +480,12 @@ class Server:
     """The serving loop (mgp_server_*)."""
 
     def __init__(self, hbm_budget=64 << 30, max_active=4, relaxation=GAUSS_SEIDEL, nsmooth=2, omega=1.0,
-                 coarse_sweeps=32, maxiter=100, tolerance=1e-8):
+                 coarse_sweeps=32, maxiter=100, tolerance=1e-8, flags=0):
         require_gpu()
         c = ServerConfig()
         c.hbm_budget, c.max_active, c.relaxation = int(hbm_budget), int(max_active), int(relaxation)
         c.nsmooth, c.omega, c.coarse_sweeps = int(nsmooth), float(omega), int(coarse_sweeps)
-        c.maxiter, c.tolerance = int(maxiter), float(tolerance)
+        c.maxiter, c.tolerance, c.flags = int(maxiter), float(tolerance), int(flags)
         h = vp()
         check(lib().mgp_server_create(ctypes.byref(c), ctypes.byref(h)), "mgp_server_create")
         self.h = h.value

@@ -66,6 +66,9 @@ public:
                                      // with FP64 arithmetic (sets coarse_fp64 of its FP32 levels)
     bool coarse_fp64 = false;      // FP32 levels: the coarsest level's Gauss-Seidel passes in FP64
                                    // arithmetic where the level fits one workgroup's LDS
+    bool coarse_cg = false;        // CONSISTENT only: each visit of the coarsest level solves it by
+                                   // CG in FP64 (to 1e-8 relative) where it fits one workgroup,
+                                   // instead of coarse_sweeps smoothing passes
 
     Settings() {}
     Settings(const Settings& s) = default;

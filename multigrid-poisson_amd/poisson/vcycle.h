@@ -5,6 +5,8 @@
 #ifndef POISSON_VCYCLE
 #define POISSON_VCYCLE
 
+#include <type_traits>
+
 #include "gaussseidel.h"
 #include "jacobi.h"
 #include "residual.h"
@@ -43,6 +45,21 @@ void descend(Domain<T>*** domains, Restriction<T>& restriction, uint_t level, ui
     }
 }
 
+// addition: the coarsest level solved by CG in FP64 (mgp_cg_solve_*), in place
+// on u; false where the level does not fit one workgroup (smoothing then)
+template <class T>
+bool coarse_cg_solve(Domain<T>& d) {
+    int64_t b[6];
+    d.box(b);
+    const mgp_array u = d.u->desc(), f = d.f->desc();
+    const mgp_faces F = d.faces();
+    const int rc = std::is_same<T, float>::value ? mgp_cg_solve_f32(&u, &f, &F, b, 2000, 1e-8, mgpx::stream())
+                                                 : mgp_cg_solve_f64(&u, &f, &F, b, 2000, 1e-8, mgpx::stream());
+    if (rc == MGP_ERR_ARG) return false;
+    mgpx::check(rc, "coarsest-level CG");
+    return true;
+}
+
 // vcycle.h:81-94: u_l += P(u_{l+1}) with r_l as the scratch
 template <class T>
 void ascend(Domain<T>*** domains, Prolongation<T>& prolongation, uint_t level, uint_t num_devices) {
@@ -68,8 +85,9 @@ void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& 
             return;
         }
         // coarsest level: smoothing only (vcycle.h:27-31); in CONSISTENT mode
-        // optionally more passes, a stronger coarsest-level solve
+        // optionally more passes, or a CG solve of the level (coarse_cg)
         const Settings& st = d.settings;
+        if (st.mode == CONSISTENT && st.coarse_cg && detail::coarse_cg_solve(d)) return;
         const int_t n = (st.mode == CONSISTENT && st.coarse_sweeps > 0) ? st.coarse_sweeps : nsmooth;
         relaxation.relax(domains, omega, n, level, num_devices);
         return;
