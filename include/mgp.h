@@ -480,14 +480,16 @@ int mgp_batch_tile(void* batch, int64_t p, int64_t level, int64_t out[2]);
  * admitted in arrival order while fewer than max_active run and their
  * hierarchy fits in hbm_budget next to the active and resident ones;
  * hierarchies stay resident for later requests of the same shape and face
- * types (least recently used evicted first); each active solve has its own
- * stream and the worker advances every solve one refinement iteration at a
- * time, so their work overlaps on the device.  A result is the same bits
+ * types (least recently used evicted first).  max_active worker threads
+ * each run one solve at a time on a stream of their own, so the solves' work
+ * overlaps on the device.  Hierarchies are carved from one device region of
+ * hbm_budget * 9/8 bytes (within free memory) allocated at creation, so new
+ * shapes and evictions make no hipMalloc / hipFree.  A result is the same bits
  * whether its request ran alone or among others.  All-Neumann requests are
  * rejected (singular).  No counterpart in the reference. */
 typedef struct {
     int64_t hbm_budget;     /* bytes of device memory for active + resident hierarchies */
-    int32_t max_active;     /* solves in flight at once (one stream each) */
+    int32_t max_active;     /* solves in flight at once (worker threads, one stream each) */
     int32_t relaxation;     /* MGP_GAUSS_SEIDEL or MGP_JACOBI */
     int64_t nsmooth;        /* smoothing passes (pre = post) */
     double omega;           /* <= 0: the smoother's default */
@@ -501,6 +503,9 @@ typedef struct {
  * and requests of the same shape); without it every kernel is launched
  * directly, which is cheaper for hierarchies used for a few iterations only */
 #define MGP_SERVER_GRAPHS 1
+/* allocate each hierarchy array by array with hipMalloc (and free it with
+ * hipFree on eviction) instead of from the server's device region */
+#define MGP_SERVER_DIRECT_ALLOC 2
 typedef struct {
     int64_t dims[3];                   /* 2^k m + 1 per axis; h = 1/(dims[0]-1) */
     int32_t bc[6];                     /* MGP_DIRICHLET / MGP_NEUMANN, order N S E W T B */

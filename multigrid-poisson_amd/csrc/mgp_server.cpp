@@ -2,10 +2,10 @@
 // types accepted as they arrive and run concurrently on one GPU.  No
 // counterpart in the reference, which runs one solve per process.
 //
-// max_active worker threads each own one stream slot at a time.  A worker
-// takes the oldest pending request once the HBM budget allows its hierarchy
-// (a RefinementSolver: FP32 F-cycles inside FP64 iterative refinement, with
-// its own stream), solves it with the same solve() a lone caller would run,
+// max_active worker threads, each with a stream of its own.  A worker takes
+// the oldest pending request once the HBM budget allows its hierarchy (a
+// RefinementSolver: FP32 F-cycles inside FP64 iterative refinement, on the
+// worker's stream), solves it with the same solve() a lone caller would run,
 // copies the solution to the caller's buffer and takes the next request.
 // Hierarchies stay resident after their request ends and are reused by later
 // requests of the same shape and face types; idle ones are evicted least
@@ -16,8 +16,15 @@
 // request's arithmetic does not depend on what else runs (own arrays and
 // stream, deterministic kernels), so its result is the same bits alone or in
 // a mix.
+//
+// Hierarchies are carved from one device region allocated when the server
+// starts (and their pinned scalars from one pinned host region), so building
+// one for a new shape and evicting an idle one make no hipMalloc / hipFree:
+// hipFree waits for the whole device, and with a mix of shapes evictions
+// were frequent enough to stall every running solve.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <condition_variable>
@@ -79,7 +86,7 @@ struct SlotBase {
     size_t bytes = 0;
     Job* job = nullptr;
     virtual ~SlotBase() {}
-    virtual void begin(const mgp_request& r) = 0;  // f, u = 0 (waits)
+    virtual void begin(const mgp_request& r, void* stream) = 0;  // on `stream`: f, u = 0 (waits)
     virtual void run() = 0;                         // the solve (waits)
     virtual void finish(double* out) = 0;           // solution to the host (waits)
     virtual void stats(int64_t* iters, double* rel) = 0;
@@ -123,7 +130,10 @@ struct Slot : SlotBase {
         nsmooth = c.nsmooth;
         omega = c.omega;
     }
-    void begin(const mgp_request& r) override {
+    void begin(const mgp_request& r, void* stream) override {
+        // an idle hierarchy has no queued work: it moves to this worker's
+        // stream without waiting for the stream it last ran on
+        solver->use_stream(stream, false);
         Poisson::Domain<double>& d = solver->fp64().get_domain(0, 0);
         const mgp_array f = d.f->desc();
         Poisson::mgpx::check(mgp_fill_modes_f64(&f, r.nmodes, r.abc, r.amp, st.h, solver->get_stream()), "rhs");
@@ -160,7 +170,16 @@ public:
         if (cfg.omega <= 0.0) cfg.omega = c.relaxation == MGP_JACOBI ? 6.0 / 7.0 : 8.0 / 9.0;
         int dev = 0;
         hip_check(hipGetDevice(&dev), "server");
-        for (int w = 0; w < c.max_active; w++) workers.emplace_back([this, dev] { run(dev); });
+        if (!(c.flags & MGP_SERVER_DIRECT_ALLOC)) make_arenas();
+        streams.resize((size_t)c.max_active, nullptr);
+        try {
+            for (auto& st : streams) hip_check(hipStreamCreateWithFlags((hipStream_t*)&st, hipStreamNonBlocking),
+                                               "server stream");
+        } catch (...) {
+            release();
+            throw;
+        }
+        for (int w = 0; w < c.max_active; w++) workers.emplace_back([this, dev, w] { run(dev, streams[(size_t)w]); });
     }
     ~Server() {
         {
@@ -169,7 +188,8 @@ public:
         }
         cv_work.notify_all();
         for (auto& t : workers) t.join();
-        idle.clear();
+        idle.clear();  // every slot's work has finished (workers wait for it)
+        release();
     }
 
     int64_t submit(const mgp_request& r) {
@@ -217,6 +237,10 @@ public:
 
 private:
     mgp_server_config cfg;
+    std::vector<void*> streams;  // one per worker, destroyed after every slot
+    void* dev_region = nullptr;
+    void* host_region = nullptr;
+    std::unique_ptr<Poisson::mgpx::Arena> dev_arena, host_arena;
     std::vector<std::thread> workers;
     std::mutex mu;  // guards everything below
     std::condition_variable cv_work, cv_done;
@@ -227,6 +251,43 @@ private:
     std::list<std::unique_ptr<SlotBase>> idle;  // resident, least recently used first
     size_t bytes_used = 0;                      // active + resident hierarchies (and reservations)
     int64_t active = 0, completed = 0, peak_active = 0;
+
+    // the device region: the budget plus an eighth for the arena's
+    // fragmentation, within what the device has free (halved until it can be
+    // allocated; without it hierarchies are allocated one array at a time)
+    void make_arenas() {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return;
+        const size_t margin = (size_t)1 << 30;
+        size_t want = (size_t)cfg.hbm_budget + (size_t)cfg.hbm_budget / 8;
+        if (free_b > margin) want = std::min(want, free_b - margin);
+        for (; want >= ((size_t)256 << 20); want /= 2)
+            if (hipMalloc(&dev_region, want) == hipSuccess) break;
+        (void)hipGetLastError();
+        if (!dev_region) return;
+        dev_arena.reset(new Poisson::mgpx::Arena(dev_region, want));
+        Poisson::mgpx::register_arena(dev_arena.get());
+        const size_t host_bytes = (size_t)4 << 20;  // 16384 pinned scalars
+        if (hipHostMalloc(&host_region, host_bytes, hipHostMallocDefault) == hipSuccess) {
+            host_arena.reset(new Poisson::mgpx::Arena(host_region, host_bytes));
+            Poisson::mgpx::register_arena(host_arena.get());
+        } else {
+            host_region = nullptr;
+            (void)hipGetLastError();
+        }
+    }
+    void release() {
+        for (auto& st : streams)
+            if (st) (void)hipStreamDestroy((hipStream_t)st);
+        streams.clear();
+        if (dev_arena) Poisson::mgpx::unregister_arena(dev_arena.get());
+        if (host_arena) Poisson::mgpx::unregister_arena(host_arena.get());
+        dev_arena.reset();
+        host_arena.reset();
+        if (dev_region) (void)hipFree(dev_region);
+        if (host_region) (void)hipHostFree(host_region);
+        dev_region = host_region = nullptr;
+    }
 
     static Key key_of(const mgp_request& r) {
         uint32_t bits = 0;
@@ -242,8 +303,10 @@ private:
     }
 
     // under mu: the oldest request and a resident slot for it, or a
-    // reservation of its bytes for a new slot; false when nothing can start
-    bool take(Job*& j, std::unique_ptr<SlotBase>& s, size_t& reserved) {
+    // reservation of its bytes for a new slot; false when nothing can start.
+    // Idle slots evicted for room go to `evicted`, destroyed outside the lock.
+    bool take(Job*& j, std::unique_ptr<SlotBase>& s, size_t& reserved,
+              std::vector<std::unique_ptr<SlotBase>>& evicted) {
         if (pending.empty()) return false;
         j = pending.front();
         const Key k = key_of(j->req);
@@ -258,7 +321,8 @@ private:
         const size_t need = need_of(j->req);
         while (bytes_used + need > (size_t)cfg.hbm_budget && !idle.empty()) {
             bytes_used -= idle.front()->bytes;
-            idle.pop_front();  // frees its device arrays
+            evicted.push_back(std::move(idle.front()));
+            idle.pop_front();
         }
         // wait for room unless nothing else holds memory (then try anyway)
         if (bytes_used + need > (size_t)cfg.hbm_budget && active > 0) return false;
@@ -280,19 +344,25 @@ private:
         cv_done.notify_all();
     }
 
-    void run(int dev) {
+    void run(int dev, void* stream) {
         (void)hipSetDevice(dev);
+        // this worker's hierarchies: carved from the server's regions and
+        // enqueued on its stream
+        Poisson::mgpx::ArenaScope arenas(dev_arena.get(), host_arena.get());
+        Poisson::mgpx::AdoptStream adopt(stream);
         while (true) {
             Job* j = nullptr;
             std::unique_ptr<SlotBase> s;
             size_t reserved = 0;
+            std::vector<std::unique_ptr<SlotBase>> evicted;
             {
                 std::unique_lock<std::mutex> lk(mu);
-                cv_work.wait(lk, [&] { return (stopping && pending.empty()) || take(j, s, reserved); });
+                cv_work.wait(lk, [&] { return (stopping && pending.empty()) || take(j, s, reserved, evicted); });
                 if (!j) return;  // stopping, nothing pending
                 active++;
                 peak_active = std::max(peak_active, active);
             }
+            evicted.clear();  // idle: no queued work, blocks back to the arena
             j->t_admit = Clock::now();
             int status = MGP_OK;
             try {
@@ -303,7 +373,7 @@ private:
                     // same estimate, need_of) and leaves with it
                     reserved = 0;
                 }
-                s->begin(j->req);
+                s->begin(j->req, stream);
                 j->res.setup_s = seconds(j->t_admit, Clock::now());
                 s->run();
                 const auto t0 = Clock::now();
@@ -318,7 +388,11 @@ private:
                              e.what());
                 status = MGP_ERR_STATE;
             }
-            // a slot whose request failed may hold a broken stream or graph:
+            // nothing of this request stays queued (a failed one may have
+            // left work behind): its slot's blocks can go back to the arena
+            (void)hipStreamSynchronize((hipStream_t)stream);
+            (void)hipGetLastError();
+            // a slot whose request failed may hold a broken graph or state:
             // it is destroyed, not kept for later requests
             std::unique_ptr<SlotBase> broken;
             {

@@ -66,6 +66,7 @@ class ServerConfig(ctypes.Structure):
 
 
 SERVER_GRAPHS = 1
+SERVER_DIRECT_ALLOC = 2
 
 
 class Request(ctypes.Structure):

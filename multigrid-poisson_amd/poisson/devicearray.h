@@ -62,14 +62,14 @@ DeviceArray<T>::DeviceArray(Settings& settings, Halo& _halo)
 
 template <class T>
 DeviceArray<T>::~DeviceArray() {
-    mgp_free(this->devptr);
+    mgpx::dev_free(this->devptr);
 }
 
 template <class T>
 void DeviceArray<T>::device_allocator() {
     mgpx::require_device();  // devicearray.h:117-119
     void* p = nullptr;
-    mgpx::check(mgp_malloc(&p, (size_t)this->size * sizeof(T)), "Error allocating DeviceArray");
+    mgpx::check(mgpx::dev_alloc(&p, (size_t)this->size * sizeof(T)), "Error allocating DeviceArray");
     this->devptr = static_cast<T*>(p);
 }
 
@@ -137,22 +137,26 @@ namespace detail {
 struct Scalar {
     double* dev = nullptr;
     double* host = nullptr;
-    Scalar() {
+    // direct: never from an arena (a thread's scalar lives until the thread
+    // ends, which may be after an arena installed on the thread is gone)
+    explicit Scalar(bool direct = false) {
+        mgpx::ArenaScope scope(direct ? nullptr : mgpx::device_arena_slot(),
+                               direct ? nullptr : mgpx::host_arena_slot());
         void* p = nullptr;
-        mgpx::check(mgp_malloc(&p, 8 * sizeof(double)), "scalar alloc");
+        mgpx::check(mgpx::dev_alloc(&p, 8 * sizeof(double)), "scalar alloc");
         dev = static_cast<double*>(p);
-        mgpx::check(mgp_host_alloc(&p, 8 * sizeof(double)), "scalar alloc");
+        mgpx::check(mgpx::host_alloc(&p, 8 * sizeof(double)), "scalar alloc");
         host = static_cast<double*>(p);
     }
     ~Scalar() {
-        mgp_free(dev);
-        mgp_host_free(host);
+        mgpx::dev_free(dev);
+        mgpx::host_free(host);
     }
     Scalar(const Scalar&) = delete;
     Scalar& operator=(const Scalar&) = delete;
 };
 inline Scalar& thread_scalar() {
-    thread_local Scalar s;
+    thread_local Scalar s(true);
     return s;
 }
 }  // namespace detail

@@ -173,7 +173,7 @@ Domain<T>::Domain(Settings& _settings, BoundaryCondition& BC, bool duplicate, in
     this->south = make(ref ? DIRICHLET : BC.south, SOUTH, nx, 1, nz);
     if (!requires_duplicate && settings.fused_sweeps) {
         void* p = nullptr;
-        mgpx::check(mgp_malloc(&p, (size_t)u->size * sizeof(T)), "Domain: u_alt");
+        mgpx::check(mgpx::dev_alloc(&p, (size_t)u->size * sizeof(T)), "Domain: u_alt");
         u_alt = static_cast<T*>(p);
     }
 }
@@ -191,7 +191,7 @@ Domain<T>::~Domain() {
         delete this->west;
         delete this->top;
         delete this->bottom;
-        mgp_free(u_alt);
+        mgpx::dev_free(u_alt);
     }
 }
 

@@ -37,7 +37,7 @@ public:
         int64_t n[3] = {last.u->shape[0], last.u->shape[1], last.u->shape[2]};
         double h = last.settings.h;
         void* p = nullptr;
-        mgpx::check(mgp_malloc(&p, (count + 1) * sizeof(double)), "Fp16Levels");
+        mgpx::check(mgpx::dev_alloc(&p, (count + 1) * sizeof(double)), "Fp16Levels");
         norms = static_cast<double*>(p);
         try {
             for (uint_t l = 0; l < count; l++) {
@@ -114,7 +114,7 @@ private:
 
     static void* alloc(const Level& v) {
         void* p = nullptr;
-        mgpx::check(mgp_malloc(&p, (size_t)(v.n[0] * v.n[1] * v.n[2]) * 2), "Fp16Levels");
+        mgpx::check(mgpx::dev_alloc(&p, (size_t)(v.n[0] * v.n[1] * v.n[2]) * 2), "Fp16Levels");
         return p;
     }
     static mgp_array desc(const Level& v, void* data) {
@@ -134,12 +134,12 @@ private:
     }
     void release() {
         for (Level& v : lv) {
-            mgp_free(v.u);
-            mgp_free(v.f);
-            mgp_free(v.r);
+            mgpx::dev_free(v.u);
+            mgpx::dev_free(v.f);
+            mgpx::dev_free(v.r);
         }
         lv.clear();
-        if (norms) mgp_free(norms);
+        if (norms) mgpx::dev_free(norms);
         norms = nullptr;
     }
 

@@ -121,6 +121,7 @@ public:
     void init(funptr ufun, funptr ffun, funptr dudxfun, funptr dudyfun, funptr dudzfun);
     Domain<T>& get_domain(uint_t gpuid, uint_t level) { return *domains[gpuid][level]; }
     void* get_stream() const { return stream; }
+    bool owns_its_stream() const { return owns_stream; }
     uint_t get_levels() const { return settings.levels; }
     const Settings& get_settings() const { return settings; }
     // one cycle ("Vcycle"/"Fcycle") without norms, enqueued (graph-replayed
@@ -136,9 +137,13 @@ public:
     }
     // enqueue on `s` (owned by the caller) instead of this solver's own stream;
     // recorded cycle graphs are dropped
-    void use_stream(void* s) {
-        mgp_stream_sync(stream);
+    // (wait = false: the caller knows the current stream holds none of this
+    // solver's work)
+    void use_stream(void* s, bool wait = true) {
+        if (s == stream) return;
+        if (wait) mgp_stream_sync(stream);
         clear_graphs();
+        if (owns_stream) mgp_stream_destroy(stream);
         stream = s;
         owns_stream = false;
     }
@@ -176,6 +181,9 @@ PoissonSolver<T, R, P, S>::PoissonSolver(int_t _dev_shape[3], Settings& _setting
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>
 PoissonSolver<T, R, P, S>::~PoissonSolver() {
+    // queued work first, then the arrays it uses (an adopted stream is the
+    // adopter's to drain: it may carry other solvers' work)
+    if (stream && owns_stream) mgp_stream_sync(stream);
     clear_graphs();
     if (domains) {
         for (uint_t gpuid = 0; gpuid < num_devices; gpuid++) {
@@ -186,10 +194,7 @@ PoissonSolver<T, R, P, S>::~PoissonSolver() {
         delete[] domains;
     }
     delete scalar;
-    if (stream) {
-        mgp_stream_sync(stream);
-        if (owns_stream) mgp_stream_destroy(stream);
-    }
+    if (stream && owns_stream) mgp_stream_destroy(stream);
 }
 
 // libpoisson.h:113-156 on one device
@@ -199,7 +204,12 @@ void PoissonSolver<T, R, P, S>::alloc() {
         throw std::invalid_argument("PoissonSolver: dev_shape must be 1x1x1 (one GPU per solver)");
     mgpx::require_device();
     mgpx::check(mgp_set_device((int)settings.dev), "PoissonSolver: set device");
-    mgpx::check(mgp_stream_create(&stream), "PoissonSolver: stream");
+    if (void* s = mgpx::adopted_stream_slot()) {
+        stream = s;
+        owns_stream = false;
+    } else {
+        mgpx::check(mgp_stream_create(&stream), "PoissonSolver: stream");
+    }
     mgpx::StreamScope scope(stream);
     scalar = new detail::Scalar();
     domains = new Domain<T>**[1];

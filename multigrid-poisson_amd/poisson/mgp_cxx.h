@@ -6,6 +6,7 @@
 #include <stdexcept>
 #include <string>
 
+#include "arena.h"
 #include "definitions.h"
 #include "mgp.h"
 
@@ -48,6 +49,24 @@ struct StreamScope {
     void* prev;
     explicit StreamScope(void* s) : prev(current_stream_slot()) { current_stream_slot() = s; }
     ~StreamScope() { current_stream_slot() = prev; }
+};
+
+// A stream for the solvers constructed on this thread to enqueue on instead
+// of creating their own (owned by whoever installs it, e.g. a server worker
+// that runs many solvers in turn); nullptr: each solver creates its own.  A
+// solver on an adopted stream does not wait for it when destroyed: the
+// adopter makes sure the solver's queued work has finished first.
+inline void*& adopted_stream_slot() {
+    thread_local void* s = nullptr;
+    return s;
+}
+
+struct AdoptStream {
+    void* prev;
+    explicit AdoptStream(void* s) : prev(adopted_stream_slot()) { adopted_stream_slot() = s; }
+    ~AdoptStream() { adopted_stream_slot() = prev; }
+    AdoptStream(const AdoptStream&) = delete;
+    AdoptStream& operator=(const AdoptStream&) = delete;
 };
 
 inline void require_device() {

@@ -88,22 +88,25 @@ public:
             }
         }
         void* p = nullptr;
-        mgpx::check(mgp_malloc(&p, 8 * sizeof(double)), "RefinementSolver");
+        mgpx::check(mgpx::dev_alloc(&p, 8 * sizeof(double)), "RefinementSolver");
         dev_norm = static_cast<double*>(p);
-        mgpx::check(mgp_host_alloc(&p, 8 * sizeof(double)), "RefinementSolver");
+        mgpx::check(mgpx::host_alloc(&p, 8 * sizeof(double)), "RefinementSolver");
         host_norm = static_cast<double*>(p);
         if (s.refine_hilo) alloc_hilo();
     }
     RefinementSolver(const RefinementSolver&) = delete;
     RefinementSolver& operator=(const RefinementSolver&) = delete;
     ~RefinementSolver() {
+        // the FP32 levels' arrays are in use by work queued on the FP64
+        // solver's stream (not waited for here when the stream is adopted)
+        if (outer->owns_its_stream()) mgp_stream_sync(outer->get_stream());
         delete inner;
         delete half;
         delete outer;
-        mgp_free(dev_norm);
-        mgp_host_free(host_norm);
-        for (void* q : lo) mgp_free(q);
-        mgp_free(own_alt);
+        mgpx::dev_free(dev_norm);
+        mgpx::host_free(host_norm);
+        for (void* q : lo) mgpx::dev_free(q);
+        mgpx::dev_free(own_alt);
     }
 
     void init() { this->init(&ufun, &ffun, &dudxfun, &dudyfun, &dudzfun); }
@@ -169,6 +172,11 @@ public:
         mgpx::StreamScope scope(st);
         Domain<double>& d64 = outer->get_domain(0, 0);
         fnorm = d64.f->infinity_norm();
+        // r = 0 first: the residual kernels write the box points only, and the
+        // FP32 right-hand side is cast from all of r, boundary nodes included,
+        // which the F-cycle's restriction of f reads (a caller may have used
+        // r as scratch between solves, as the server does for results)
+        d64.r->init_zero();
         // a solve starts from the FP64 u as stored: lo = 0
         if (lo[0]) {
             const mgp_array l = lo_desc(cur);
@@ -283,6 +291,12 @@ public:
     PoissonSolver<float, R, P, S>& fp32() { return *inner; }
     Fp16Levels* fp16() { return half; }  // nullptr without FP16 levels
     void* get_stream() const { return outer->get_stream(); }
+    // enqueue on `s` (owned by the caller) from now on (waits for the current
+    // stream unless wait = false; recorded cycle graphs are dropped)
+    void use_stream(void* s, bool wait = true) {
+        outer->use_stream(s, wait);
+        inner->use_stream(s, false);
+    }
 
 private:
     void enqueue_norm(Domain<double>& d, void* st) {
@@ -305,13 +319,13 @@ private:
         const size_t n = (size_t)d64.u->size;
         mgpx::StreamScope scope(outer->get_stream());
         for (int k = 0; k < 2; k++) {
-            mgpx::check(mgp_malloc(&lo[k], n * sizeof(float)), "RefinementSolver: lo");
+            mgpx::check(mgpx::dev_alloc(&lo[k], n * sizeof(float)), "RefinementSolver: lo");
             const mgp_array l = lo_desc(k);
             mgpx::check(mgp_fill_f32(&l, 0.0, outer->get_stream()), "RefinementSolver: lo");
         }
         if (!d64.u_alt) {
             void* p = nullptr;
-            mgpx::check(mgp_malloc(&p, n * sizeof(double)), "RefinementSolver: u");
+            mgpx::check(mgpx::dev_alloc(&p, n * sizeof(double)), "RefinementSolver: u");
             own_alt = static_cast<double*>(p);
         }
         mgpx::check(mgp_stream_sync(outer->get_stream()), "RefinementSolver: lo");

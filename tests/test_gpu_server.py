@@ -79,3 +79,32 @@ def test_rejects_all_neumann_and_bad_shapes():
     with pytest.raises(mgp.MgpError):
         s.submit((2, 17, 17), "DDDDDD", [(1, 1, 1, 1.0)])
     s.close()
+
+
+def test_reused_hierarchy_gives_the_fresh_result():
+    """A request served on a resident hierarchy (same shape and face types as
+    an earlier request) equals the same request on a fresh one, bit for bit,
+    and reaches the tolerance.  The server hands results back through the
+    FP64 residual array, whose boundary nodes the refinement's FP32
+    right-hand side is cast from: a reused hierarchy once started from that
+    data and stalled near 1e-3.  Shapes and modes of requests that failed
+    so; with the server's device region and with array-by-array allocation."""
+    rng = np.random.default_rng(zlib.crc32(b"reuse"))
+    cases = [((129, 129, 33), "DNDDNN"), ((33, 65, 129), "DNNDNN"), ((65, 33, 129), "DDDDNN")]
+    for flags in (0, mgp.SERVER_DIRECT_ALLOC):
+        for dims, bc in cases:
+            first, second = ([(int(a), int(b), int(c), float(rng.standard_normal()))
+                              for a, b, c in rng.integers(1, 9, size=(4, 3))] for _ in range(2))
+            say("case reuse", dims, bc, "flags", flags)
+            s = mgp.Server(max_active=1, tolerance=1e-8, flags=flags)
+            s.wait(s.submit(dims, bc, first))
+            res, u = s.wait(s.submit(dims, bc, second))  # on the resident hierarchy
+            s.close()
+            fresh = mgp.Server(max_active=1, tolerance=1e-8, flags=flags)
+            r2, u2 = fresh.wait(fresh.submit(dims, bc, second))
+            fresh.close()
+            say("  reused: it %d rel %.3e; fresh: it %d rel %.3e" % (res.iterations, res.rel_res, r2.iterations,
+                                                                    r2.rel_res))
+            assert res.status == 0 and res.rel_res <= 1e-8
+            assert (res.iterations, res.rel_res) == (r2.iterations, r2.rel_res)
+            np.testing.assert_array_equal(u, u2)
