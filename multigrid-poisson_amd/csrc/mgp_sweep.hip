@@ -116,7 +116,7 @@ inline int complement_blocks(const View<T>& u, const int64_t b[6], int threads) 
 // order: x outside the box (whole planes), then y outside (box planes), then
 // z outside (box rows); consecutive threads take consecutive z where it runs
 template <class T>
-__device__ void copy_complement(const MarchArgs<T>& A, int blk) {
+__device__ __forceinline__ void copy_complement(const MarchArgs<T>& A, int blk) {
     const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2;
     const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
     const int64_t xo = A.bx0 + n0 - A.bx1, yo = A.by0 + n1 - A.by1, zo = A.bz0 + n2 - A.bz1;

@@ -543,7 +543,10 @@ def test_lds_multi_sweep_rejects_large_levels():
     assert rc == mgp.ERR_ARG
 
 
-SWEEP2_SHAPES = SWEEP_SHAPES + [((3, 40, 200), "NNDDNN"), ((21, 3, 3), "NNNNNN"), ((12, 57, 113), "DDNNDD")]
+SWEEP2_SHAPES = SWEEP_SHAPES + [((3, 40, 200), "NNDDNN"), ((21, 3, 3), "NNNNNN"), ((12, 57, 113), "DDNNDD"),
+                                # several interior tiles in y and z next to the edge ones
+                                ((30, 40, 400), "NNDDDD"), ((33, 26, 371), "DDNNDD"), ((24, 21, 490), "NDNDNN"),
+                                ((26, 17, 250), "DDDDDD")]
 # (dtype, sweep2.rows, sweep.target_wgs): several x-chunks per tile, and one
 SWEEP2_CASES = [("f64", 16, 16384), ("f64", 12, 16384), ("f64", 16, 1), ("f32", 20, 16384), ("f32", 16, 16384),
                 ("f32", 24, 1)]
