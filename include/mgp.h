@@ -282,6 +282,13 @@ int mgp_bc_dirichlet_f32(const mgp_array* u, int loc, const void* g, void* strea
  * WEST/SOUTH/BOTTOM (include/neumann.h:63-131); g NULL = homogeneous */
 int mgp_bc_neumann_f64(const mgp_array* u, int loc, const void* g, double h, void* stream);
 int mgp_bc_neumann_f32(const mgp_array* u, int loc, const void* g, double h, void* stream);
+/* The ghost layers of every face of type MGP_NEUMANN in `faces` in one launch
+ * (faces->g[q], NULL = homogeneous; faces->h): the boundary refresh before a
+ * residual (include/residual.h:26-34, include/neumann.h:133-144 per face).  The
+ * faces write disjoint ghost points and read inside the shape, so this equals
+ * mgp_bc_neumann_* on each Neumann face in any order. */
+int mgp_bc_neumann_faces_f64(const mgp_array* u, const mgp_faces* faces, void* stream);
+int mgp_bc_neumann_faces_f32(const mgp_array* u, const mgp_faces* faces, void* stream);
 /* Coarse Neumann data as the reference forms it (include/neumann.h:146-252):
  * g_coarse = R(g_fine) - (c1 u(b) + c2 u(b+-1) + c3 u(b+-2))/h_fine with
  * c = (-3/2, 2, -1/2) on WEST/SOUTH/BOTTOM and (3/2, -2, 1/2) elsewhere;
@@ -384,7 +391,9 @@ int mgp_solver_stats(void* solver, double* rel_res, double* seconds, int64_t* it
  * MGP_SOLVER_PLAIN_ITERATE) */
 int mgp_solver_array(void* solver, int level, int which, void** host, mgp_array* device);
 /* host mirror and device descriptor of a face array; *type receives the
- * effective boundary type (MGP_DIRICHLET / MGP_NEUMANN) */
+ * effective boundary type (MGP_DIRICHLET / MGP_NEUMANN).  The face arrays of
+ * levels >= 1 belong to the solver: each descent sets them (zero in consistent
+ * mode, where they are filled once and kept), so a caller writes level 0's */
 int mgp_solver_face(void* solver, int level, int loc, void** host, mgp_array* device, int* type);
 /* VTK of level 0's u (save, libpoisson.h:359-363) */
 int mgp_solver_save(void* solver, const char* path);

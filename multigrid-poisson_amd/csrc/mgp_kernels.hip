@@ -608,6 +608,43 @@ __global__ void k_bc_neumann(View<T> u, int loc, const T* g, T two_h) {
     u.at(a, b, c) = sign * two_h * gv + u.at(a + d0, b + d1, c + d2);
 }
 
+// The ghosts of every Neumann face in one launch: thread t belongs to the
+// face whose range [start[q], start[q+1]) holds it and does what k_bc_neumann
+// does for that face.  Each face writes only its own ghost layer (the face's
+// points, edges of the halo excluded) and reads points inside the shape, so
+// the faces do not depend on one another and the result equals the per-face
+// refreshes in any order.
+template <class T>
+struct NeumannFaces {
+    const T* g[6];
+    int64_t start[7];  // prefix of the Neumann faces' point counts (0 for the others)
+    T two_h;
+};
+
+template <class T>
+__global__ void k_bc_neumann_faces(View<T> u, NeumannFaces<T> F) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= F.start[6]) return;
+    int loc = 0;
+    while (t >= F.start[loc + 1]) loc++;
+    int64_t f0, f1, f2;
+    face_geometry(loc, u.n0, u.n1, u.n2, f0, f1, f2);
+    const int64_t p = t - F.start[loc];
+    const int64_t i = p / (f1 * f2), j = (p / f2) % f1, k = p % f2;
+    int64_t a = i, b = j, c = k, d0 = 0, d1 = 0, d2 = 0;
+    T sign = 1.0;
+    switch (loc) {
+    case MGP_EAST: a = u.n0; d0 = -2; break;
+    case MGP_WEST: a = -1; d0 = 2; sign = -1.0; break;
+    case MGP_NORTH: b = u.n1; d1 = -2; break;
+    case MGP_SOUTH: b = -1; d1 = 2; sign = -1.0; break;
+    case MGP_TOP: c = u.n2; d2 = -2; break;
+    default: c = -1; d2 = 2; sign = -1.0; break;
+    }
+    const T gv = F.g[loc] ? F.g[loc][p] : (T)0.0;
+    u.at(a, b, c) = sign * F.two_h * gv + u.at(a + d0, b + d1, c + d2);
+}
+
 // include/neumann.h:146-252: coarse face data = R(fine face data), then minus
 // the one-sided difference of the fine solution at the face divided by h_fine.
 template <class T>
@@ -1177,6 +1214,26 @@ static int bc_neumann(const mgp_array* u, int loc, const void* g, double h, void
     return launch_status();
 }
 
+template <class T>
+static int bc_neumann_faces(const mgp_array* u, const mgp_faces* faces, void* stream) {
+    if (!valid_array(u) || !faces) return MGP_ERR_ARG;
+    NeumannFaces<T> F{};
+    F.start[0] = 0;
+    for (int q = 0; q < 6; q++) {
+        const bool nm = faces->type[q] == MGP_NEUMANN;
+        if (nm && halo_of_face(u, q) < 1) return MGP_ERR_ARG;
+        F.g[q] = nm ? static_cast<const T*>(faces->g[q]) : nullptr;
+        F.start[q + 1] = F.start[q] + (nm ? face_points(q, u) : 0);
+    }
+    if (F.start[6] == 0) return MGP_OK;
+    for (int d = 0; d < 3; d++)
+        if (u->shape[d] < 3) return MGP_ERR_ARG;  // the mirror point must exist
+    F.two_h = (T)(2.0 * faces->h);  // include/neumann.h:108
+    hipLaunchKernelGGL(k_bc_neumann_faces<T>, dim3((unsigned)((F.start[6] + 255) / 256)), dim3(256), 0,
+                       (hipStream_t)stream, view_of<T>(*u), F);
+    return launch_status();
+}
+
 static inline mgp_array face_array(void* data, int loc, const int64_t n[3]) {
     mgp_array a{};
     a.data = data;
@@ -1557,6 +1614,12 @@ int mgp_bc_neumann_f64(const mgp_array* u, int l, const void* g, double h, void*
 }
 int mgp_bc_neumann_f32(const mgp_array* u, int l, const void* g, double h, void* s) {
     return bc_neumann<float>(u, l, g, h, s);
+}
+int mgp_bc_neumann_faces_f64(const mgp_array* u, const mgp_faces* faces, void* s) {
+    return bc_neumann_faces<double>(u, faces, s);
+}
+int mgp_bc_neumann_faces_f32(const mgp_array* u, const mgp_faces* faces, void* s) {
+    return bc_neumann_faces<float>(u, faces, s);
 }
 int mgp_bc_restrict_neumann_f64(int r, const mgp_array* u, int l, const void* gf, void* gc, double h, void* s) {
     return bc_restrict_neumann<double>(r, u, l, gf, gc, h, s);

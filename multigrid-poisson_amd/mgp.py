@@ -144,6 +144,7 @@ for sfx in ("f64", "f32"):
         "mgp_restrict_inject_" + sfx: [P(Array), P(Array), vp], "mgp_restrict_fw_" + sfx: [P(Array), P(Array), vp],
         "mgp_prolong_trilinear_" + sfx: [P(Array), P(Array), vp], "mgp_prolong_add_" + sfx: [P(Array), P(Array), vp],
         "mgp_bc_dirichlet_" + sfx: [P(Array), i32, vp, vp], "mgp_bc_neumann_" + sfx: [P(Array), i32, vp, d, vp],
+        "mgp_bc_neumann_faces_" + sfx: [P(Array), P(Faces), vp],
         "mgp_bc_restrict_neumann_" + sfx: [i32, P(Array), i32, vp, vp, d, vp],
     })
 

@@ -37,6 +37,16 @@ protected:
     Boundary<T>* neighbor = nullptr;  // internal (device-to-device) faces only
     Location_t location;
     void init_by_fun(funptr fun, Settings& settings);
+    // addition: the device copy of arr is known to be all zero (set where
+    // this class zeroes it, cleared where anything else may write it); a
+    // coarse face whose data is zero at every descent is then filled once,
+    // not at every descent
+    bool zero_on_device = false;
+    void zero_once() {
+        if (zero_on_device && this->arr.is_on_device()) return;
+        this->arr.init_zero();
+        zero_on_device = this->arr.is_on_device();
+    }
 
 public:
     DeviceArray<T> arr;
@@ -56,6 +66,14 @@ public:
     virtual bool is_non_eliminated() = 0;
 
     Location_t get_location() const { return location; }  // addition
+    // addition: a coarse level's face data as every descent's restrict_to
+    // leaves it where that is always zero (Dirichlet faces; Neumann faces in
+    // consistent mode), set before the first descent so that recorded cycles
+    // need no fill at all
+    void prepare_coarse(const Settings& s) {
+        if (is_internal_boundary()) return;
+        if (is_non_eliminated() || s.mode == CONSISTENT) zero_once();
+    }
 };
 
 template <class T>

@@ -31,11 +31,17 @@ public:
     void restrict_to(DeviceArray<T>& u, Boundary<T>& boundary, Settings& settings,
                      Restriction<T>& restriction) override {
         (void)u, (void)boundary, (void)settings, (void)restriction;
-        this->arr.init_zero();
+        this->zero_once();
     }
     bool is_internal_boundary() override { return false; }
-    void init_zero() override { this->arr.init_zero(); }
-    void to_device() override { this->arr.to_device(); }
+    void init_zero() override {
+        this->arr.init_zero();
+        this->zero_on_device = this->arr.is_on_device();
+    }
+    void to_device() override {
+        this->arr.to_device();
+        this->zero_on_device = false;
+    }
     void link(Boundary<T>* _boundary) override { (void)_boundary; }
     bool is_non_eliminated() override { return true; }
 };

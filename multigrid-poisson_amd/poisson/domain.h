@@ -126,6 +126,10 @@ public:
         return F;
     }
     void swap_u_alt() { std::swap(u->devptr, u_alt); }
+    // a coarse level's face data before the first descent (Boundary::prepare_coarse)
+    void prepare_coarse_faces() {
+        for (int q = 0; q < 6; q++) face(q)->prepare_coarse(settings);
+    }
     // u (and uprev) = 0 with the boundary data written again, on the device;
     // the state to_device() leaves for a fresh solve with unchanged f and faces
     void reset_solution() {

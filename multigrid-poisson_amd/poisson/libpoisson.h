@@ -253,6 +253,7 @@ void PoissonSolver<T, R, P, S>::to_device_zero() {
     mgpx::StreamScope scope(stream);
     clear_graphs();
     for (uint_t l = 0; l < settings.levels; l++) domains[0][l]->to_device_zero();
+    for (uint_t l = 1; l < settings.levels; l++) domains[0][l]->prepare_coarse_faces();
 }
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>
@@ -260,6 +261,7 @@ void PoissonSolver<T, R, P, S>::to_device() {
     mgpx::StreamScope scope(stream);
     clear_graphs();
     for (uint_t l = 0; l < settings.levels; l++) domains[0][l]->to_device();
+    for (uint_t l = 1; l < settings.levels; l++) domains[0][l]->prepare_coarse_faces();
 }
 
 template <class T, template <class> class R, template <class> class P, template <class> class S>

@@ -131,6 +131,9 @@ template <class T> struct Ops;
         static int bc_neumann(const mgp_array* u, int l, const void* g, double h, void* s) {                     \
             return mgp_bc_neumann_##SFX(u, l, g, h, s);                                                           \
         }                                                                                                         \
+        static int bc_neumann_faces(const mgp_array* u, const mgp_faces* F, void* s) {                           \
+            return mgp_bc_neumann_faces_##SFX(u, F, s);                                                           \
+        }                                                                                                         \
         static int bc_restrict_neumann(int r, const mgp_array* u, int l, const void* gf, void* gc, double h,     \
                                        void* s) {                                                                \
             return mgp_bc_restrict_neumann_##SFX(r, u, l, gf, gc, h, s);                                          \

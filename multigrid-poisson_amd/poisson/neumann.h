@@ -33,9 +33,10 @@ public:
     void restrict_to(DeviceArray<T>& u, Boundary<T>& boundary, Settings& settings,
                      Restriction<T>& restriction) override {
         if (settings.mode == CONSISTENT) {
-            this->arr.init_zero();
+            this->zero_once();  // homogeneous coarse data
             return;
         }
+        this->zero_on_device = false;
         const int kind = restriction.mgp_kind();
         if (kind < 0)
             throw std::invalid_argument("Neumann::restrict_to: reference-mode coarse Neumann data needs a built-in restriction");
@@ -45,8 +46,14 @@ public:
                     "Neumann::restrict_to");
     }
     bool is_internal_boundary() override { return false; }
-    void init_zero() override { this->arr.init_zero(); }
-    void to_device() override { this->arr.to_device(); }
+    void init_zero() override {
+        this->arr.init_zero();
+        this->zero_on_device = this->arr.is_on_device();
+    }
+    void to_device() override {
+        this->arr.to_device();
+        this->zero_on_device = false;
+    }
     void link(Boundary<T>* _boundary) override { (void)_boundary; }
     bool is_non_eliminated() override { return false; }
 };

@@ -9,6 +9,16 @@ namespace Poisson {
 namespace detail {
 template <class T>
 inline void refresh_boundaries(Domain<T>& domain, bool fetch_neighbor) {
+    // one device: Dirichlet updates are no-ops and the Neumann faces'
+    // refreshes are independent, so all of them go in one launch
+    bool single = true;
+    for (int q = 0; q < 6; q++) single = single && !domain.face(q)->is_internal_boundary();
+    if (single) {
+        const mgp_array u = domain.u->desc();
+        const mgp_faces F = domain.faces();
+        mgpx::check(mgpx::Ops<T>::bc_neumann_faces(&u, &F, mgpx::stream()), "boundary refresh");
+        return;
+    }
     domain.east->update(domain, false, fetch_neighbor);
     domain.west->update(domain, false, fetch_neighbor);
     domain.north->update(domain, false, fetch_neighbor);

@@ -878,3 +878,23 @@ def test_norm_only_residual(shape, bc, dtype):
         mgp.call("mgp_residual_norm_" + sfx, mode, u.ref(), f.ref(), r2.ref(), mgp.box(box), h, n2.ptr, None)
         assert n1.download()[0] == n2.download()[0] > 0
         np.testing.assert_array_equal(r2.download(), r0)
+
+
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,bc", [((9, 17, 65), "NNNNDD"), ((17, 15, 13), "DNNDDN"), ((5, 5, 5), "NNNNNN"),
+                                      ((33, 9, 12), "NDNDND"), ((7, 6, 5), "DDDDDD")], ids=["0", "1", "2", "3", "4"])
+def test_neumann_faces_in_one_launch_equal_per_face_refreshes(shape, bc, dtype):
+    """mgp_bc_neumann_faces (every Neumann face's ghost layer in one launch) ==
+    mgp_bc_neumann on each Neumann face, over the whole halo-padded array,
+    halo edges and corners (written by neither) included."""
+    say("case neumann faces", shape, bc, dtype.__name__)
+    sfx = "f64" if dtype == np.float64 else "f32"
+    lv = random_level(shape, bc, mgp.CONSISTENT, dtype, (shape, bc, "nfaces"))
+    halo, u0, f, faces, F, box = lv
+    a = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    b = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    for q in range(6):
+        if bc[q] == "N":
+            mgp.call("mgp_bc_neumann_" + sfx, a.ref(), q, faces[q].ptr, F.h, None)
+    mgp.call("mgp_bc_neumann_faces_" + sfx, b.ref(), ctypes.byref(F), None)
+    np.testing.assert_array_equal(b.download(), a.download())
