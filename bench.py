@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--traffic-probe", action="store_true",
                     help="run once each: a fill (write-only) and an inf-norm (read-only) of one 4.3 GB array, "
                          "then one fine-level one-pass sweep of the workload; no JSON line (for rocprofv3 --pmc)")
+    ap.add_argument("--probe-batch", action="store_true",
+                    help="run the batched leg's solve launch by launch (for profilers); no JSON line")
     ap.add_argument("--probe-mixed", type=int, default=0, metavar="PROBLEM",
                     help="two FP32-refinement solves of problem 2 or 3 launch by launch (for kernel traces); "
                          "no JSON line")
@@ -110,6 +112,9 @@ def main():
         return
     if args.probe_mixed:
         probe_mixed(args, args.probe_mixed)
+        return
+    if args.probe_batch:
+        probe_batch(args)
         return
     for kv in args.tune:
         k, v = kv.split("=")
@@ -407,6 +412,28 @@ def probe_mixed(args, problem):
         log("probe mixed p%d solve %d: %.1f ms, %d iterations, rel_res %.3e" %
             (problem, k, 1e3 * (time.perf_counter() - t0), it, rel))
     s.close()
+
+
+def probe_batch(args):
+    """The batched leg's solve (same problems and settings as batch_leg), one
+    warm-up and one measured solve, launch by launch (for kernel traces and
+    counters); no JSON line."""
+    import numpy as np
+
+    rng = np.random.default_rng(args.seed + 4)
+    n = args.batch_count
+    dims = [tuple(int(v) for v in rng.choice([33, 65, 129], size=3)) for _ in range(n)]
+    modes = [modes_of(rng) for _ in range(n)]
+    b = mgp.Batch(dims, relaxation=mgp.JACOBI, nsmooth=3, omega=6.0 / 7.0, maxiter=60, tolerance=args.tol,
+                  flags=mgp.BATCH_NO_GRAPHS)
+    for p in range(n):
+        b.set_rhs(p, modes_rhs(dims[p], modes[p]))
+    for k in range(2):
+        t = b.solve()
+        st = [b.stats(p) for p in range(n)]
+        log("probe batch solve %d: %.1f ms, iterations %d..%d, worst rel_res %.2e" %
+            (k, 1e3 * t, min(i for i, _ in st), max(i for i, _ in st), max(r for _, r in st)))
+    b.close()
 
 
 def guarded_leg(name, skip, fn):
