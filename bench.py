@@ -77,6 +77,8 @@ def main():
     ap.add_argument("--coarse-sweeps", type=int, default=32,
                     help="smoothing passes per visit of the coarsest level (17x17x9)")
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--coarse-cg", action="store_true",
+                    help="solve the coarsest level by CG (MGP_SOLVER_COARSE_CG) in the FP64 and mixed legs")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-graphs", action="store_true", help="launch kernels one by one (for profilers)")
     ap.add_argument("--tune", action="append", default=[], metavar="KEY=VALUE",
@@ -125,7 +127,7 @@ def main():
     dof = n * n * nz
     log("bench: %dx%dx%d, %d levels, F-cycle GS(%d) FW, tol %.0e" % (n, n, nz, args.levels, args.nsmooth, args.tol))
     s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
-                   mgp.FULL_WEIGHTING, problem=2, flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0),
+                   mgp.FULL_WEIGHTING, problem=2, flags=solver_flags(args),
                    maxiter=100, miniter=1, tolerance=args.tol,
                    coarse_sweeps=args.coarse_sweeps)
     s.to_device()
@@ -211,7 +213,7 @@ def main():
         "config": {"workload": "poisson-fcycle-%dx%dx%d" % dims, "grid": list(dims), "levels": args.levels,
                    "cycle": "F", "smoother": "red-black GS SOR omega=%g, %d sweeps" % (args.omega, args.nsmooth),
                    "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
-                   "coarsest_sweeps": args.coarse_sweeps,
+                   "coarsest": "CG (FP64, one workgroup)" if args.coarse_cg else "%d GS passes" % args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "rbgs_sweep2 (fine level, two sweeps per pass)",
                      "achieved": pair_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pair_gbs / HBM_PEAK_GBS,
@@ -239,6 +241,11 @@ def measured_traffic(n):
     except OSError:
         pass
     return None
+
+
+def solver_flags(args):
+    return (mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0) |
+            (mgp.COARSE_CG if args.coarse_cg else 0))
 
 
 def timed_solves(s, args):
@@ -289,7 +296,7 @@ def mixed_leg(dims, h, args):
             log("bench: mixed leg, %s %s" % (key, name))
             s = mgp.Solver(dims, args.levels, h, (0.0, 0.0, 0.0), bc, mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
                            mgp.FULL_WEIGHTING, problem=prob, dtype=dt,
-                           flags=mgp.QUIET | (mgp.NO_GRAPHS if args.no_graphs else 0), maxiter=30, miniter=1,
+                           flags=solver_flags(args), maxiter=30, miniter=1,
                            tolerance=tol, coarse_sweeps=args.coarse_sweeps, half_from=half_from)
             s.to_device()
             el, cyc, rel = timed_solves(s, args)

@@ -493,7 +493,8 @@ int mgp_batch_tile(void* batch, int64_t p, int64_t level, int64_t out[2]);
  * each run one solve at a time on a stream of their own, so the solves' work
  * overlaps on the device.  Hierarchies are carved from one device region of
  * hbm_budget * 9/8 bytes (within free memory) allocated at creation, so new
- * shapes and evictions make no hipMalloc / hipFree.  A result is the same bits
+ * shapes and evictions make no hipMalloc / hipFree (an array that finds no
+ * free range large enough in the region is allocated on its own).  A result is the same bits
  * whether its request ran alone or among others.  All-Neumann requests are
  * rejected (singular).  No counterpart in the reference. */
 typedef struct {
