@@ -139,16 +139,20 @@ def main():
     if max(rel) > args.tol:
         log("bench: WARNING: tolerance not reached")
 
-    # Dominant kernels measured live on the fine level: the two-sweep pass the
-    # smoother runs (24 B per box point: read u and f once, write u once, for
-    # two sweeps), the one-pass sweep (24 B per point per sweep) and the
-    # residual with its max-norm (24 B per point: read u and f, write r).
+    # Dominant kernels measured live on the fine level: the two-sweep pass with
+    # the coarse correction added first, which the F-cycle runs twice per
+    # cycle on the fine level (24 B per box point: read u and f once, write u
+    # once, for two sweeps, plus 8 B per coarse point read once), the plain
+    # two-sweep pass (once per cycle), the one-pass sweep (24 B per point per
+    # sweep) and the residual with its max-norm (24 B per point: read u and f,
+    # write r).
     L = mgp.lib()
     stream = mgp.vp()
     mgp.check(L.mgp_stream_create(ctypes.byref(stream)))
     u = s.device_array("u")
     f_ = s.device_array("f")
     r = s.device_array("r")
+    u1 = s.device_array("u", 1)
     F = mgp.Faces()
     for q in range(6):
         t = s.face_type(q)
@@ -172,17 +176,36 @@ def main():
         lambda st: mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r),
                                                     ctypes.byref(f_), ctypes.byref(F), bx, 1.0, st)),
         args.kernel_reps, stream)
+    pro_ms = kernel_time_ms(
+        lambda st: mgp.check(L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(r),
+                                                            ctypes.byref(f_), ctypes.byref(F), bx, 1.0,
+                                                            ctypes.byref(u1), st)), args.kernel_reps, stream)
     resid_ms = kernel_time_ms(
         lambda st: mgp.check(L.mgp_residual_f64(mgp.CONSISTENT, ctypes.byref(u), ctypes.byref(f_), ctypes.byref(r),
                                                  bx, h, scal.ptr, st)), args.kernel_reps, stream)
     L.mgp_stream_sync(stream)
     sweep_bytes = 24.0 * box_pts
+    pro_bytes = 24.0 * box_pts + 8.0 * u1.shape[0] * u1.shape[1] * u1.shape[2]
     resid_bytes = 24.0 * box_pts
     sweep_gbs = sweep_bytes / (sweep_ms * 1e-3) / 1e9
     pair_gbs = sweep_bytes / (pair_ms * 1e-3) / 1e9
+    pro_gbs = pro_bytes / (pro_ms * 1e-3) / 1e9
     resid_gbs = resid_bytes / (resid_ms * 1e-3) / 1e9
-    log("bench: sweep %.2f ms (%.0f GB/s), two-sweep pass %.2f ms (%.0f GB/s), residual %.2f ms (%.0f GB/s)" %
-        (sweep_ms, sweep_gbs, pair_ms, pair_gbs, resid_ms, resid_gbs))
+    log("bench: sweep %.2f ms (%.0f GB/s), two-sweep pass %.2f ms (%.0f GB/s), with the correction %.2f ms "
+        "(%.0f GB/s), residual %.2f ms (%.0f GB/s)" %
+        (sweep_ms, sweep_gbs, pair_ms, pair_gbs, pro_ms, pro_gbs, resid_ms, resid_gbs))
+    # fine-level launches per F-cycle: three smoothing passes, the two after a
+    # correction with the correction inside where sweep2.prolong is on
+    fused = ctypes.c_int64(0)
+    mgp.check(L.mgp_tuning_get(b"sweep2.prolong", ctypes.byref(fused)))
+    n_pro = 2 if fused.value >= 1 else 0
+    pair_kernel = {"kernel": "rbgs_sweep2 (fine level, two sweeps per pass)", "achieved": pair_gbs,
+                   "bytes_per_launch": sweep_bytes, "ms_per_launch": pair_ms, "launches_per_cycle": 3 - n_pro,
+                   "traffic": measured_traffic(n, "rbgs_sweep2")}
+    pro_kernel = {"kernel": "rbgs_sweep2_prolong (fine level, correction u += P(u_coarse) and two sweeps per pass)",
+                  "achieved": pro_gbs, "bytes_per_launch": pro_bytes, "ms_per_launch": pro_ms,
+                  "launches_per_cycle": n_pro, "traffic": measured_traffic(n, "rbgs_sweep2_prolong")}
+    top = max((pair_kernel, pro_kernel), key=lambda k: k["ms_per_launch"] * k["launches_per_cycle"])
 
     s.close()
     mixed = guarded_leg("mixed", args.no_mixed, lambda: mixed_leg(dims, h, args))
@@ -215,10 +238,15 @@ def main():
                    "restriction": "full weighting", "bc": "Neumann x,y / Dirichlet z", "tolerance": args.tol,
                    "coarsest": "CG (FP64, one workgroup)" if args.coarse_cg else "%d GS passes" % args.coarse_sweeps,
                    "cycles_to_tol": cycles[-1], "final_rel_res": rel[-1], "parallelism": "single GPU"},
-        "roofline": {"bound": "hbm", "kernel": "rbgs_sweep2 (fine level, two sweeps per pass)",
-                     "achieved": pair_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pair_gbs / HBM_PEAK_GBS,
-                     "traffic": measured_traffic(n), "bytes_per_launch": sweep_bytes, "ms_per_launch": pair_ms},
-        "kernels": {"rbgs_sweep": {"ms": sweep_ms, "GB/s": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS},
+        "roofline": {"bound": "hbm", "kernel": top["kernel"], "achieved": top["achieved"], "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": top["achieved"] / HBM_PEAK_GBS, "traffic": top["traffic"],
+                     "bytes_per_launch": top["bytes_per_launch"], "ms_per_launch": top["ms_per_launch"],
+                     "launches_per_cycle": top["launches_per_cycle"]},
+        "kernels": {"rbgs_sweep2": {"ms": pair_ms, "GB/s": pair_gbs, "frac": pair_gbs / HBM_PEAK_GBS,
+                                    "launches_per_cycle": pair_kernel["launches_per_cycle"]},
+                    "rbgs_sweep2_prolong": {"ms": pro_ms, "GB/s": pro_gbs, "frac": pro_gbs / HBM_PEAK_GBS,
+                                            "launches_per_cycle": n_pro},
+                    "rbgs_sweep": {"ms": sweep_ms, "GB/s": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS},
                     "residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
         "fp32_refine": mixed,
         "batched": batch,
@@ -229,11 +257,12 @@ def main():
     print(json.dumps(out), flush=True)
 
 
-def measured_traffic(n):
-    """HBM bytes per fine-level two-sweep pass from the rocprofv3 FETCH_SIZE /
-    WRITE_SIZE runs summarised in profiles/ (scaled by calibration kernels of
-    known size), for the 1025x1025x513 workload; None for other sizes."""
-    path = os.path.join(ROOT, "profiles", "rbgs_sweep2_%dx%dx%d_traffic.md" % (n, n, n // 2 + 1))
+def measured_traffic(n, kernel):
+    """HBM bytes per fine-level launch of `kernel` (rbgs_sweep2 or
+    rbgs_sweep2_prolong) from the rocprofv3 FETCH_SIZE / WRITE_SIZE runs
+    summarised in profiles/ (scaled by calibration kernels of known size), for
+    the 1025x1025x513 workload; None for other sizes or without a summary."""
+    path = os.path.join(ROOT, "profiles", "%s_%dx%dx%d_traffic.md" % (kernel, n, n, n // 2 + 1))
     try:
         for line in open(path):
             if line.startswith("traffic_bytes_per_launch:"):
@@ -383,6 +412,7 @@ def traffic_probe(args):
     mgp.check(L.mgp_add_f64(R(r), R(f_), None))
     mgp.check(L.mgp_rbgs_sweep_f64(mgp.CONSISTENT, R(u), R(r), R(f_), R(F), box, 1.0, None))
     mgp.check(L.mgp_rbgs_sweep2_f64(mgp.CONSISTENT, R(u), R(r), R(f_), R(F), box, 1.0, None))
+    mgp.check(L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, R(u), R(r), R(f_), R(F), box, 1.0, R(u1), None))
     mgp.check(L.mgp_residual_f64(mgp.CONSISTENT, R(u), R(f_), R(r), box, h, scal.ptr, None))
     mgp.check(L.mgp_residual_norm_f64(mgp.CONSISTENT, R(u), R(f_), R(r), box, h, scal.ptr, None))
     mgp.check(L.mgp_residual_restrict_fw_f64(mgp.CONSISTENT, R(u), R(f_), R(r), R(f1), box, h, None))
@@ -392,10 +422,12 @@ def traffic_probe(args):
     L.mgp_stream_sync(None)
     pts = n * n * nz
     bpts = n * n * (nz - 2)
-    log("traffic probe (algorithmic bytes): fill %d, norm %d, add %d, sweep %d, two-sweep pass %d, residual %d, "
-        "norm-only residual %d, residual + FW %d, prolongation + add %d, hi+lo update + residual %d" %
-        (8 * pts, 8 * pts, 24 * pts, 24 * bpts, 24 * bpts, 24 * bpts, 16 * bpts, 16 * pts + 8 * pts // 8,
-         16 * pts + 8 * pts // 8, 40 * pts))
+    cpts = u1.shape[0] * u1.shape[1] * u1.shape[2]
+    log("traffic probe (algorithmic bytes): fill %d, norm %d, add %d, sweep %d, two-sweep pass %d, two-sweep pass "
+        "with the correction %d, residual %d, norm-only residual %d, residual + FW %d, prolongation + add %d, "
+        "hi+lo update + residual %d" %
+        (8 * pts, 8 * pts, 24 * pts, 24 * bpts, 24 * bpts, 24 * bpts + 8 * cpts, 24 * bpts, 16 * bpts,
+         16 * pts + 8 * pts // 8, 16 * pts + 8 * pts // 8, 40 * pts))
     s.close()
 
 

@@ -206,6 +206,21 @@ int mgp_rbgs_sweep2_f64(int mode, const mgp_array* u_in, const mgp_array* u_out,
                         const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
 int mgp_rbgs_sweep2_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                         const mgp_faces* faces, const int64_t box[6], double omega, void* stream);
+/* The cycle's correction and the two sweeps after it in one pass: u_out =
+ * GS(GS(u_in + P(coarse))) on the box and u_in + P(coarse) at the shape
+ * points outside it, bit for bit mgp_prolong_add_* of coarse into u_in
+ * followed by mgp_rbgs_sweep2_* (the prolongation and correction of
+ * include/vcycle.h:81-94 and the first two post-smoothing sweeps; no single
+ * counterpart in the reference).  u_in is not changed.  Consistent mode only;
+ * coarse 2:1 nested in u_in (shape 2(n-1)+1 per axis) and neither u_in nor
+ * u_out; otherwise, and where mgp_rbgs_sweep2_* refuses the level,
+ * MGP_ERR_ARG. */
+int mgp_rbgs_sweep2_prolong_f64(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                                const mgp_faces* faces, const int64_t box[6], double omega,
+                                const mgp_array* coarse, void* stream);
+int mgp_rbgs_sweep2_prolong_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
+                                const mgp_faces* faces, const int64_t box[6], double omega,
+                                const mgp_array* coarse, void* stream);
 /* nsweeps such sweeps in one launch, for a level small enough for one
  * workgroup's LDS ((nx+2)(ny+2)(nz+2) + nx ny nz elements <= 64 KiB;
  * MGP_ERR_ARG otherwise): the result of the last sweep goes to u_out, which may
@@ -584,7 +599,11 @@ int mgp_cycle_noalloc_f64(const char* cycle, const mgp_array* u, const mgp_array
  * (register rows per two-sweep
  * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16, default),
  * "sweep2.chunks_per_tile" (least x chunks per two-sweep wave tile, made odd,
- * and at least enough for about 6800 waves; default 15), "residual.target_wgs",
+ * and at least enough for about 6800 waves; default 15), "sweep2.prolong" (1:
+ * in consistent mode the cycles add the prolongation of the coarse correction
+ * inside the first two-sweep pass after it, mgp_rbgs_sweep2_prolong_*, on FP64
+ * levels, default; 2: on FP32 levels too; 0: mgp_prolong_add_* and then the
+ * sweeps), "residual.target_wgs",
  * "residual_fw.target_wgs", "prolong.target_wgs" (waves the norm-only
  * residual, the fused residual + full-weighting and the prolongation marches
  * aim for; 0 = the value of

@@ -982,13 +982,22 @@ static int rbgs_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const 
     return march_sweep<T>(mode, ui, uo, f, faces, b, omega, march_xchunk(b), stream);
 }
 
-// two sweeps in one pass, u_out = GS(GS(u_in)), u_out a different buffer
+// two sweeps in one pass, u_out = GS(GS(u_in)), u_out a different buffer;
+// with coarse, u_out = GS(GS(u_in + P(coarse))) (consistent mode)
 template <class T>
 static int rbgs_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,
-                       const mgp_faces* faces, const int64_t b[6], double omega, void* stream) {
+                       const mgp_faces* faces, const int64_t b[6], double omega, void* stream,
+                       const mgp_array* coarse = nullptr) {
     if (!valid_array(ui) || !valid_array(uo) || !valid_array(f) || !faces || !same_shape(ui, uo) ||
         !same_shape(ui, f) || ui->data == uo->data || (mode != MGP_MODE_REFERENCE && mode != MGP_MODE_CONSISTENT))
         return MGP_ERR_ARG;
+    if (coarse) {
+        if (mode != MGP_MODE_CONSISTENT || !valid_array(coarse) || coarse->data == ui->data ||
+            coarse->data == uo->data)
+            return MGP_ERR_ARG;
+        for (int d = 0; d < 3; d++)  // exact 2:1 nesting (trilinearinterpolation.h:31-37)
+            if (2 * (coarse->shape[d] - 1) + 1 != ui->shape[d]) return MGP_ERR_ARG;
+    }
     if (!checkerboard(mode, ui) || lin0_of(ui) != lin0_of(uo) || ui->stride[0] != uo->stride[0] ||
         ui->stride[1] != uo->stride[1])
         return MGP_ERR_ARG;
@@ -1005,7 +1014,7 @@ static int rbgs_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const
     for (int q = 0; q < 6; q++)
         if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
     if (box_empty(bx)) return MGP_ERR_ARG;  // u_out would not receive the copy of u_in
-    return march_sweep2<T>(mode, ui, uo, f, faces, b, omega, stream);
+    return march_sweep2<T>(mode, ui, uo, f, faces, b, omega, stream, coarse);
 }
 
 // LDS bytes the multi-sweep kernel needs for a level, 0 when it does not fit
@@ -1562,6 +1571,16 @@ int mgp_rbgs_sweep2_f64(int m, const mgp_array* ui, const mgp_array* uo, const m
 int mgp_rbgs_sweep2_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                         const int64_t b[6], double w, void* s) {
     return rbgs_sweep2<float>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep2_prolong_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,
+                                const mgp_faces* F, const int64_t b[6], double w, const mgp_array* c, void* s) {
+    if (!c) return MGP_ERR_ARG;
+    return rbgs_sweep2<double>(m, ui, uo, f, F, b, w, s, c);
+}
+int mgp_rbgs_sweep2_prolong_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,
+                                const mgp_faces* F, const int64_t b[6], double w, const mgp_array* c, void* s) {
+    if (!c) return MGP_ERR_ARG;
+    return rbgs_sweep2<float>(m, ui, uo, f, F, b, w, s, c);
 }
 int mgp_rbgs_sweeps_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                         const int64_t b[6], double w, int64_t n, void* s) {

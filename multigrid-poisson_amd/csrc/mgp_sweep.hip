@@ -84,6 +84,7 @@ constexpr int NBORDER = 2 * UW + 2 * RH;  // outermost ring of the old-plane rec
 template <class T>
 struct MarchArgs {
     View<T> ui, uo, f;
+    View<T> c;  // coarse level whose prolongation the pass adds first (two-sweep pass only)
     const T* g[6];
     T two_h;
     int64_t bx0, bx1, by0, by1, bz0, bz1;
@@ -114,8 +115,9 @@ inline int complement_blocks(const View<T>& u, const int64_t b[6], int threads) 
 
 // points [blk * blockDim * COPY_PER_THREAD, ...) of the complement, in the
 // order: x outside the box (whole planes), then y outside (box planes), then
-// z outside (box rows); consecutive threads take consecutive z where it runs
-template <class T>
+// z outside (box rows); consecutive threads take consecutive z where it runs.
+// PRO: ui + P(coarse) instead of ui (the pass that adds the prolongation)
+template <class T, bool PRO = false>
 __device__ __forceinline__ void copy_complement(const MarchArgs<T>& A, int blk) {
     const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2;
     const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
@@ -141,7 +143,10 @@ __device__ __forceinline__ void copy_complement(const MarchArgs<T>& A, int blk) 
             y = A.by0 + r / zo;
             z = c < A.bz0 ? c : A.bz1 + (c - A.bz0);
         }
-        A.uo.at(x, y, z) = A.ui.at(x, y, z);
+        if (PRO)
+            A.uo.at(x, y, z) = A.ui.at(x, y, z) + trilinear<T>(A.c, x, y, z);
+        else
+            A.uo.at(x, y, z) = A.ui.at(x, y, z);
     }
 }
 
@@ -444,13 +449,28 @@ __global__ void __launch_bounds__(64 * (RH / NQ)) k_rbgs_march(MarchArgs<T> A) {
 // half-sweep's stencil reads.
 constexpr int WZ = 60;  // interior columns per wave tile
 
-template <class T>
-__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1 (z - 1)
-    return __builtin_amdgcn_update_dpp((T)0, v, 0x138, 0xf, 0xf, false);  // wave_shr:1
+// DPP lane move of a 32- or 64-bit value with bound_ctrl: a lane whose source
+// lane lies outside the wave reads 0.  No old value is involved, so the
+// compiler emits no initialising move before each DPP move.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_mov_dpp(__builtin_bit_cast(int, v), CTRL, 0xf, 0xf, true));
+}
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double v) {
+    typedef int i32x2 __attribute__((ext_vector_type(2)));
+    i32x2 w = __builtin_bit_cast(i32x2, v);
+    w.x = __builtin_amdgcn_mov_dpp(w.x, CTRL, 0xf, 0xf, true);
+    w.y = __builtin_amdgcn_mov_dpp(w.y, CTRL, 0xf, 0xf, true);
+    return __builtin_bit_cast(double, w);
 }
 template <class T>
-__device__ __forceinline__ T lane_next(T v) {  // value of lane l+1 (z + 1)
-    return __builtin_amdgcn_update_dpp((T)0, v, 0x130, 0xf, 0xf, false);  // wave_shl:1
+__device__ __forceinline__ T lane_prev(T v) {  // value of lane l-1 (z - 1); 0 on lane 0
+    return dpp_move<0x138>(v);                 // wave_shr:1
+}
+template <class T>
+__device__ __forceinline__ T lane_next(T v) {  // value of lane l+1 (z + 1); 0 on lane 63
+    return dpp_move<0x130>(v);                 // wave_shl:1
 }
 
 // Buffer-resource access (SGPR descriptor + 32-bit lane offset + SGPR row
@@ -727,9 +747,25 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
 // in-plane ghosts from the rows / lanes two apart, the x-ghost planes -1 and
 // n0 from the planes two apart.  Per point the arithmetic is the single
 // sweep's, so the result equals two sweeps bit for bit.
+//
+// PRO (mgp_rbgs_sweep2_prolong_*): the pass first adds the trilinear
+// prolongation of a coarse level to every plane as it arrives, so the cycle's
+// correction u += P(u_coarse) (include/vcycle.h:81-94) and the two sweeps after
+// it take one pass over u.  The correction is formed with the operations of
+// mgp_prolong_add_* (x-, then y-, then z-midpoints, each 0.5 * (left + right),
+// include/trilinearinterpolation.h:50-133), so the values are the same.  Each
+// lane keeps one coarse column q at rows cy0 .. cy0 + NR/2 of the two coarse
+// planes under the fine plane j (Ca: j >> 1, Cb: (j >> 1) + 1); an odd z takes
+// its second column from the neighbouring lane.  PY is the parity of y at
+// register row 0 and PZ that of z at lane 0 (the launch fixes both: tiles step
+// by even counts): PZ = 0 has each lane read column (z + 1) >> 1, so an odd z
+// finds column z >> 1 on the lane before; PZ = 1 has it read z >> 1 and find
+// (z >> 1) + 1 on the lane after.  Rows, lanes and planes outside the shape
+// receive values that no update reads (they are masked, or ghosts formed
+// afresh before every stage).
 constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
 
-template <class T, int MODE, int NR, bool EDGE>
+template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0>
 __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     constexpr int RO = NR - 8;  // rows written per tile
     const int tz = tl % A.ztiles;
@@ -781,6 +817,61 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
         const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.f.p + min(max(j, (int64_t)0), n0 - 1) * s0, plane_bytes);
 #pragma unroll
         for (int t = 1; t <= NR - 2; t++) F[t] = bload<T>(r, zoff, roff[t]);
+    };
+
+    // PRO: coarse column q (clamped into the coarse shape) of rows cy0 + b;
+    // the row goes into the lane offset, so rows outside the coarse shape
+    // (edge tiles only; no update reads what they feed) fall outside the
+    // plane's record range and read 0
+    constexpr int CR = NR / 2 + 1;
+    T Ca[CR], Cb[CR];
+    int32_t cvoff = 0, crow = 0;
+    uint32_t cplane = 0;
+    if constexpr (PRO) {
+        const int64_t c1 = A.c.n1, c2 = A.c.n2;
+        const int64_t cy0 = (y0 - 4 - PY) >> 1;
+        const int64_t q = min(max((z + 1 - PZ) >> 1, (int64_t)0), c2 - 1);
+        crow = (int32_t)(A.c.s1 * (int64_t)sizeof(T));
+        cvoff = (int32_t)(cy0 * crow + q * (int64_t)sizeof(T));
+        cplane = (uint32_t)(((c1 - 1) * A.c.s1 + c2) * (int64_t)sizeof(T));
+    }
+    auto load_coarse = [&](int64_t a, T (&V)[CR]) {
+        const __amdgpu_buffer_rsrc_t r =
+            plane_rsrc(A.c.p + min(max(a, (int64_t)0), A.c.n0 - 1) * A.c.s0, cplane);
+#pragma unroll
+        for (int b = 0; b < CR; b++) V[b] = bload<T>(r, (uint32_t)(cvoff + b * crow), 0);
+    };
+    // Ca, Cb for fine plane j, given them for plane j - 1
+    auto next_coarse = [&](int64_t j) {
+        if (j & 1) return;
+#pragma unroll
+        for (int b = 0; b < CR; b++) Ca[b] = Cb[b];
+        load_coarse((j >> 1) + 1, Cb);
+    };
+    // P[t] += P(coarse) from the x-stage values X (coarse rows cy0 + b): the
+    // y-midpoint on odd rows, then the z-midpoint 0.5 * (Y + o) with o the
+    // other column's Y on an odd z and Y itself on an even z (0.5 * (Y + Y) is
+    // Y; a quad permutation fetches both cases in one move: PZ = 1 pairs lanes
+    // 2k <- 2k+1, PZ = 0 lanes 2k+1 <- 2k)
+    auto add_rows = [&](const T (&X)[CR], T (&P)[NR]) {
+#pragma unroll
+        for (int t = 0; t < NR; t++) {
+            const int b = (t + PY) >> 1;
+            const T Y = ((t + PY) & 1) ? half_sum<T>(X[b], X[b + 1 < CR ? b + 1 : b]) : X[b];
+            const T o = dpp_move<PZ ? 0xF5 : 0xA0>(Y);  // quad_perm [1,1,3,3] / [0,0,2,2]
+            P[t] = P[t] + half_sum<T>(Y, o);
+        }
+    };
+    // P[t] += P(coarse) at (j, y0 - 4 + t, z)
+    auto add_prolong = [&](int64_t j, T (&P)[NR]) {
+        if (j & 1) {
+            T X[CR];
+#pragma unroll
+            for (int b = 0; b < CR; b++) X[b] = half_sum<T>(Ca[b], Cb[b]);
+            add_rows(X, P);
+        } else {
+            add_rows(Ca, P);
+        }
     };
 
     // stage S at plane p (S even: red, odd: black) with west / centre / east
@@ -857,6 +948,16 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     load_plane(i0, P0);
     load_plane(i0 + 1, Pp1);
     load_f(i0, F0);
+    if constexpr (PRO) {
+        const int64_t j0 = i0 - 1;
+        load_coarse(j0 >> 1, Ca);
+        load_coarse((j0 >> 1) + 1, Cb);
+        add_prolong(j0, Pm1);
+        next_coarse(j0 + 1);
+        add_prolong(j0 + 1, P0);
+        next_coarse(j0 + 2);
+        add_prolong(j0 + 2, Pp1);
+    }
     using S0 = std::integral_constant<int, 0>;
     using S1 = std::integral_constant<int, 1>;
     using S2 = std::integral_constant<int, 2>;
@@ -864,6 +965,7 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     for (int64_t i = i0; i <= xb + 2; i++) {
         load_plane(i + 2, PL);
         load_f(i + 1, FL);
+        if constexpr (PRO) next_coarse(i + 2);
         // each stage from the first plane the chunk's output depends on
         if (i >= A.bx0 && i < A.bx1) stage(S0{}, i, Pm1, P0, Pp1, F0);
         if (i - 1 >= max(A.bx0, xa - 2) && i - 1 < A.bx1) stage(S1{}, i - 1, Pm2, Pm1, P0, Fm1);
@@ -875,6 +977,7 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
 #pragma unroll
             for (int t = 4; t <= NR - 5; t++) bstore(Pm3[t], ro, !EDGE || ybox[t] ? zoff_st : BUF_OOB, roff[t]);
         }
+        if constexpr (PRO) add_prolong(i + 2, PL);
 #pragma unroll
         for (int t = 0; t < NR; t++) {
             Pm4[t] = Pm3[t];
@@ -891,6 +994,17 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     }
 }
 
+// rows y0-3 .. y0+NR-6 (the ones stage 0 updates) inside the box, and no
+// ghost positions among the tile's rows and lanes: false for interior tiles
+template <class T, int NR>
+__device__ __forceinline__ bool sweep2_edge_tile(const MarchArgs<T>& A, int tl) {
+    const int ty = (tl / A.ztiles) % A.ytiles, tz = tl % A.ztiles;
+    const int64_t y0 = A.by0 + (int64_t)ty * (NR - 8), z0 = A.bz0 + (int64_t)tz * WZ2;
+    return y0 - 3 < A.by0 || y0 + NR - 6 >= A.by1 || (y0 - 4 < 0 && A.g[MGP_SOUTH]) ||
+           (y0 + NR - 5 >= A.ui.n1 && A.g[MGP_NORTH]) || (z0 - 4 < 0 && A.g[MGP_BOTTOM]) ||
+           (z0 + 59 >= A.ui.n2 && A.g[MGP_TOP]);
+}
+
 template <class T, int MODE, int NR>
 __global__ void __launch_bounds__(64) k_rbgs2_wave(MarchArgs<T> A) {
     if ((int)blockIdx.x >= A.ntiles_pad) {
@@ -900,15 +1014,23 @@ __global__ void __launch_bounds__(64) k_rbgs2_wave(MarchArgs<T> A) {
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (tl >= A.ntiles) return;
-    const int ty = (tl / A.ztiles) % A.ytiles, tz = tl % A.ztiles;
-    const int64_t y0 = A.by0 + (int64_t)ty * (NR - 8), z0 = A.bz0 + (int64_t)tz * WZ2;
-    // rows y0-3 .. y0+NR-6 (the ones stage 0 updates) inside the box, and no
-    // ghost positions among the tile's rows and lanes
-    const bool edge = y0 - 3 < A.by0 || y0 + NR - 6 >= A.by1 || (y0 - 4 < 0 && A.g[MGP_SOUTH]) ||
-                      (y0 + NR - 5 >= A.ui.n1 && A.g[MGP_NORTH]) || (z0 - 4 < 0 && A.g[MGP_BOTTOM]) ||
-                      (z0 + 59 >= A.ui.n2 && A.g[MGP_TOP]);
-    if (edge) wave_march2<T, MODE, NR, true>(A, tl);
+    if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MODE, NR, true>(A, tl);
     else wave_march2<T, MODE, NR, false>(A, tl);
+}
+
+// the two sweeps after u_in + P(coarse) (consistent mode); PY, PZ: parities of
+// by0 and bz0
+template <class T, int NR, int PY, int PZ>
+__global__ void __launch_bounds__(64) k_rbgs2p_wave(MarchArgs<T> A) {
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement<T, true>(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MGP_MODE_CONSISTENT, NR, true, true, PY, PZ>(A, tl);
+    else wave_march2<T, MGP_MODE_CONSISTENT, NR, false, true, PY, PZ>(A, tl);
 }
 
 
@@ -2222,17 +2344,24 @@ template int march_sweep<float>(int, const mgp_array*, const mgp_array*, const m
 int64_t g_sweep_pair = 1;   // relax_level runs sweeps in pairs through the two-sweep pass
 int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, FP32 16, 20 or 24; 0: 16
                             // (measured best for both at 1025 x 1025 x 513)
+int64_t g_sweep2_prolong = 1;  // the cycles add the prolongation inside the first post-smoothing pass
+                               // (1: FP64 levels, 2: FP32 levels too)
 
 template <class T>
 int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
-                 const int64_t b[6], double omega, void* stream) {
+                 const int64_t b[6], double omega, void* stream, const mgp_array* coarse) {
     MarchArgs<T> A;
     A.ui = view_of<T>(*ui);
     A.uo = view_of<T>(*uo);
     A.f = view_of<T>(*f);
+    A.c = coarse ? view_of<T>(*coarse) : A.ui;
     // planes addressed through buffer descriptors with a 32-bit range
     const int64_t plane_bytes = ((A.ui.n1 - 1) * A.ui.s1 + A.ui.n2) * (int64_t)sizeof(T);
     if (A.ui.n0 < 3 || plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    if (coarse) {
+        const int64_t cplane = ((A.c.n1 - 1) * A.c.s1 + A.c.n2) * (int64_t)sizeof(T);
+        if (mode != MGP_MODE_CONSISTENT || cplane >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+    }
     for (int q = 0; q < 6; q++) A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const T*>(faces->g[q]) : nullptr;
     A.two_h = (T)(2.0 * faces->h);  // include/neumann.h:108
     A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
@@ -2244,6 +2373,7 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     int rows = (int)g_sweep2_rows;
     if (sizeof(T) == 8 && rows != 12 && rows != 16) rows = 16;
     if (sizeof(T) == 4 && rows != 16 && rows != 20 && rows != 24) rows = 16;
+    if (coarse) rows = 16;  // the one tile height with the prolongation built in
     A.ytiles = (int)((b[3] - b[2] + rows - 9) / (rows - 8));
     A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
@@ -2262,6 +2392,15 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     A.ntiles_pad = (A.ntiles + 7) / 8 * 8;
     const dim3 g(A.ntiles_pad + complement_blocks(A.ui, b, 64));
     const hipStream_t st = (hipStream_t)stream;
+    if (coarse) {
+        const int py = (int)(b[2] & 1), pz = (int)(b[4] & 1);  // parities of y0 - 4 and z0 - 4
+        if (py == 0 && pz == 0) hipLaunchKernelGGL((k_rbgs2p_wave<T, 16, 0, 0>), g, dim3(64), 0, st, A);
+        else if (py == 0) hipLaunchKernelGGL((k_rbgs2p_wave<T, 16, 0, 1>), g, dim3(64), 0, st, A);
+        else if (pz == 0) hipLaunchKernelGGL((k_rbgs2p_wave<T, 16, 1, 0>), g, dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_rbgs2p_wave<T, 16, 1, 1>), g, dim3(64), 0, st, A);
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? MGP_OK : (int)e;
+    }
     const bool ref = mode == MGP_MODE_REFERENCE;
 #define MGP_WAVE2_LAUNCH(RR)                                                                            \
     if (ref) hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_REFERENCE, RR>), g, dim3(64), 0, st, A); \
@@ -2276,8 +2415,8 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     return e == hipSuccess ? MGP_OK : (int)e;
 }
 template int march_sweep2<double>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
-                                  const int64_t[6], double, void*);
+                                  const int64_t[6], double, void*, const mgp_array*);
 template int march_sweep2<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
-                                 const int64_t[6], double, void*);
+                                 const int64_t[6], double, void*, const mgp_array*);
 
 }  // namespace mgp

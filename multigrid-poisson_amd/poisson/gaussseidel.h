@@ -98,6 +98,41 @@ public:
         }
     }
 
+    // the correction and the first pair of sweeps in one pass
+    // (mgp_rbgs_sweep2_prolong_*) where relax_level would run that pair through
+    // the two-sweep pass: consistent mode, trilinear prolongation, a level of at
+    // least "sweep.pair_min_points" points, "sweep2.prolong" 1 (FP64 levels) or
+    // 2 (FP32 levels too: there the separate passes are faster, 2.47 + 1.01 ms
+    // against 4.00 ms at 1025 x 1025 x 513; FP64 4.91 + 2.04 against 5.81 ms)
+    void relax_level_prolonged(Domain<T>& d, Domain<T>& coarse, Prolongation<T>& prolongation, T omega,
+                               int_t nsmooth) override {
+        int64_t pair = 1, pair_min = 0, fuse = 1;
+        if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
+        if (mgp_tuning_get("sweep.pair_min_points", &pair_min) != MGP_OK) pair_min = 0;
+        if (mgp_tuning_get("sweep2.prolong", &fuse) != MGP_OK) fuse = 1;
+        if (!std::is_same<T, double>::value && fuse < 2) fuse = 0;
+        const bool applies = fuse && pair && d.u_alt && d.settings.fused_sweeps && d.settings.mode == CONSISTENT &&
+                             alternate && nsmooth >= 2 && prolongation.mgp_trilinear() &&
+                             d.u->shape[0] * d.u->shape[1] * d.u->shape[2] >= pair_min;
+        if (applies) {
+            int64_t b[6];
+            d.box(b);
+            const mgp_faces F = d.faces();
+            const mgp_array f = d.f->desc(), ui = d.u->desc(), c = coarse.u->desc();
+            mgp_array uo = ui;
+            uo.data = d.u_alt;
+            const int rc = mgpx::Ops<T>::rbgs_sweep2_prolong(d.settings.mode, &ui, &uo, &f, &F, b, (double)omega, &c,
+                                                             mgpx::stream());
+            if (rc == MGP_OK) {
+                d.swap_u_alt();
+                if (nsmooth > 2) relax_level(d, omega, nsmooth - 2);
+                return;
+            }
+            if (rc != MGP_ERR_ARG) mgpx::check(rc, "GaussSeidel sweep pair after the prolongation");
+        }
+        Relaxation<T>::relax_level_prolonged(d, coarse, prolongation, omega, nsmooth);
+    }
+
     int mgp_state() const override { return alternate ? 1 : 0; }
     void mgp_set_state(int s) override { alternate = s != 0; }
 };

@@ -21,6 +21,9 @@ public:
         this->prolong(coarse, scratch);
         u.add(scratch);
     }
+    // addition: true for the trilinear prolongation, which smoothers may fuse
+    // into their first pass (mgp_rbgs_sweep2_prolong_*)
+    virtual bool mgp_trilinear() const { return false; }
 };
 }  // namespace Poisson
 #endif

@@ -23,6 +23,16 @@ public:
     // ---- additions: per-level pass loop (overridable), and host-side state a
     // replayed graph must restore
     virtual void relax_level(Domain<T>& d, T omega, int_t nsmooth);
+    // the cycle's correction u += P(u_{level+1}) (include/vcycle.h:81-94) and
+    // then relax(): a smoother may add the correction inside its first pass
+    void relax_prolonged(Domain<T>*** domains, Prolongation<T>& prolongation, T omega, int_t nsmooth, int_t level,
+                         int_t num_devices);
+    // one level: prolong_add into d.u (d.r as the scratch), then relax_level
+    virtual void relax_level_prolonged(Domain<T>& d, Domain<T>& coarse, Prolongation<T>& prolongation, T omega,
+                                       int_t nsmooth) {
+        prolongation.prolong_add(*coarse.u, *d.u, *d.r);
+        relax_level(d, omega, nsmooth);
+    }
     virtual int mgp_state() const { return 0; }
     virtual void mgp_set_state(int) {}
 };
@@ -30,6 +40,18 @@ public:
 template <class T>
 void Relaxation<T>::relax(Domain<T>*** domains, T omega, int_t nsmooth, int_t level, int_t num_devices) {
     for (int_t gpuid = 0; gpuid < num_devices; gpuid++) this->relax_level(*domains[gpuid][level], omega, nsmooth);
+}
+
+template <class T>
+void Relaxation<T>::relax_prolonged(Domain<T>*** domains, Prolongation<T>& prolongation, T omega, int_t nsmooth,
+                                    int_t level, int_t num_devices) {
+    if (num_devices == 1) {
+        this->relax_level_prolonged(*domains[0][level], *domains[0][level + 1], prolongation, omega, nsmooth);
+        return;
+    }
+    for (int_t g = 0; g < num_devices; g++)
+        prolongation.prolong_add(*domains[g][level + 1]->u, *domains[g][level]->u, *domains[g][level]->r);
+    relax(domains, omega, nsmooth, level, num_devices);
 }
 
 template <class T>

@@ -16,6 +16,7 @@ public:
         mgpx::check(mgpx::Ops<T>::prolong(&a, &b, mgpx::stream()),
                     "Prolongation is incompatible with trilinear interpolation");
     }
+    bool mgp_trilinear() const override { return true; }
     void prolong_add(DeviceArray<T>& coarse, DeviceArray<T>& u, DeviceArray<T>& scratch) override {
         (void)scratch;
         const mgp_array a = coarse.desc(), b = u.desc();

@@ -60,20 +60,17 @@ bool coarse_cg_solve(Domain<T>& d) {
     return true;
 }
 
-// vcycle.h:81-94: u_l += P(u_{l+1}) with r_l as the scratch
-template <class T>
-void ascend(Domain<T>*** domains, Prolongation<T>& prolongation, uint_t level, uint_t num_devices) {
-    for (uint_t g = 0; g < num_devices; g++) {
-        Domain<T>& d = *domains[g][level];
-        Domain<T>& c = *domains[g][level + 1];
-        prolongation.prolong_add(*c.u, *d.u, *d.r);
-    }
-}
 }  // namespace detail
 
+// pending: u on `level` still lacks the correction P(u_{level+1}) of the
+// F-cycle step before (vcycle.h:81-94, deferred by Fcycle); the first smoothing
+// adds it.  The correction of this cycle's own recursion goes in with the
+// post-smoothing (Relaxation::relax_prolonged: u_l += P(u_{l+1}) with r_l as the
+// scratch, then the sweeps, or both in one pass)
 template <class T>
 void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& prolongation,
-            Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10) {
+            Relaxation<T>& relaxation, T omega, uint_t level, uint_t levels, uint_t num_devices, int_t nsmooth = 10,
+            bool pending = false) {
     if (level >= levels - 1) {
         Domain<T>& d = *domains[0][level];
         if (d.below) {  // coarser levels kept elsewhere: the same V-cycle through them
@@ -92,11 +89,11 @@ void Vcycle(Domain<T>*** domains, Restriction<T>& restriction, Prolongation<T>& 
         relaxation.relax(domains, omega, n, level, num_devices);
         return;
     }
-    relaxation.relax(domains, omega, nsmooth, level, num_devices);
+    if (pending) relaxation.relax_prolonged(domains, prolongation, omega, nsmooth, level, num_devices);
+    else relaxation.relax(domains, omega, nsmooth, level, num_devices);
     detail::descend(domains, restriction, level, num_devices);
     Vcycle<T>(domains, restriction, prolongation, relaxation, omega, level + 1, levels, num_devices, nsmooth);
-    detail::ascend(domains, prolongation, level, num_devices);
-    relaxation.relax(domains, omega, nsmooth, level, num_devices);
+    relaxation.relax_prolonged(domains, prolongation, omega, nsmooth, level, num_devices);
 }
 }  // namespace Poisson
 #endif

@@ -698,6 +698,120 @@ def test_two_sweep_pass_rejects_in_place_and_thin_levels():
                                  mgp.box([0, 2, 1, 16, 1, 16]), 1.0, None) == mgp.ERR_ARG  # Neumann without g
 
 
+# the parities of the box's first row and column (south and bottom face
+# Dirichlet or Neumann) select the kernel variant: each combination twice
+PRO_SHAPES = [((9, 17, 65), "DDDDDD"), ((17, 49, 121), "NDNDND"), ((17, 41, 129), "NNNNDD"),
+              ((33, 25, 241), "DNNDND"), ((17, 33, 129), "DNDNDN"), ((65, 41, 129), "NNNNNN"),
+              ((9, 33, 185), "NDNDDN"), ((5, 9, 17), "DDNNDN"), ((3, 5, 5), "NNDDNN")]
+# tuning: x chunks of one plane, of four planes, one chunk per tile
+PRO_CHUNKS = [dict(), dict(sweep__small_min_chunk=4), dict(sweep__small_min_chunk=1024)]
+
+
+@pytest.mark.parametrize("chunks", range(len(PRO_CHUNKS)), ids=["c1", "c4", "cx"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,bc", PRO_SHAPES, ids=[str(i) for i in range(len(PRO_SHAPES))])
+def test_two_sweep_pass_with_prolongation_equals_prolong_add_then_pass(shape, bc, dtype, chunks):
+    """mgp_rbgs_sweep2_prolong (the correction u += P(coarse) formed as each
+    plane arrives, then the two sweeps) == mgp_prolong_add into a copy of u
+    followed by mgp_rbgs_sweep2, bit for bit: the box points, the shape points
+    outside the box (u + P(coarse)), an untouched output halo and input."""
+    sfx = "f64" if dtype == np.float64 else "f32"
+    say("case sweep2 prolong", shape, bc, sfx, PRO_CHUNKS[chunks])
+    halo, u0, f, faces, F, box = random_level(shape, bc, mgp.CONSISTENT, dtype, (shape, bc, "sweep2p"))
+    cshape = tuple((n - 1) // 2 + 1 for n in shape)
+    rng = np.random.default_rng(zlib.crc32(repr((shape, bc, "coarse")).encode()))
+    c0 = rng.standard_normal(mgp.padded_shape(list(cshape), halo))
+    c = mgp.DeviceArray(cshape, halo, dtype=dtype).upload(c0)
+    a = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    b = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    other = np.random.default_rng(9).standard_normal(u0.shape)
+    got_in = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    got = mgp.DeviceArray(shape, halo, dtype=dtype).upload(other)
+    with tuned(**PRO_CHUNKS[chunks]):
+        mgp.call("mgp_prolong_add_" + sfx, c.ref(), a.ref(), None)
+        mgp.call("mgp_rbgs_sweep2_" + sfx, mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box),
+                 0.93, None)
+        mgp.call("mgp_rbgs_sweep2_prolong_" + sfx, mgp.CONSISTENT, got_in.ref(), got.ref(), f.ref(), ctypes.byref(F),
+                 mgp.box(box), 0.93, c.ref(), None)
+    want, out = b.download(), got.download()
+    inner = shape_slices(shape, halo)
+    np.testing.assert_array_equal(out[inner], want[inner])
+    outside = np.ones(out.shape, bool)
+    outside[inner] = False
+    np.testing.assert_array_equal(out[outside], other.astype(dtype)[outside])
+    np.testing.assert_array_equal(got_in.download(), u0.astype(dtype))  # the input is not written
+    np.testing.assert_array_equal(c.download(), c0.astype(dtype))
+
+
+def test_two_sweep_pass_with_prolongation_rejects_bad_coarse_levels():
+    """Reference mode, a coarse level that is not 2:1 nested, and a coarse
+    level that is the input or output buffer are refused (MGP_ERR_ARG)."""
+    shape, bc = (17, 17, 17), "DDDDDD"
+    halo, u0, f, faces, F, box = random_level(shape, bc, mgp.CONSISTENT, np.float64, (shape, bc, "reject"))
+    u = mgp.DeviceArray(shape, halo).upload(u0)
+    v = mgp.DeviceArray(shape, halo)
+    good = mgp.DeviceArray((9, 9, 9), halo)
+    bad = mgp.DeviceArray((9, 9, 8), halo)
+    L = mgp.lib()
+    args = (f.ref(), ctypes.byref(F), mgp.box(box), 1.0)
+    assert L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, u.ref(), v.ref(), *args, good.ref(), None) == mgp.OK
+    assert L.mgp_rbgs_sweep2_prolong_f64(mgp.REFERENCE, u.ref(), v.ref(), *args, good.ref(), None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, u.ref(), v.ref(), *args, bad.ref(), None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, u.ref(), v.ref(), *args, None, None) == mgp.ERR_ARG
+    assert L.mgp_rbgs_sweep2_prolong_f64(mgp.CONSISTENT, u.ref(), u.ref(), *args, good.ref(), None) == mgp.ERR_ARG
+    L.mgp_stream_sync(None)
+
+
+@pytest.mark.parametrize("flags", ["graphs", "nographs"])
+@pytest.mark.parametrize("case", [c for c in CONSISTENT_CASES if c[3] == 1 and c[6] >= 2],
+                         ids=lambda c: "%dx%dx%d_%s_%s" % (c[0] + (c[2], c[5])))
+def test_prolongation_in_sweep_pass_matches_oracle_bitwise(case, flags):
+    """Consistent-mode cycles with sweep pairs on every level (so every
+    correction goes into the first two-sweep pass after it) equal the oracle
+    bit for bit over four cycles: relative residuals, u and r."""
+    say("case prolongation in sweep pass", case, flags)
+    dims, levels, bc, relax, restr, cyc, nsm, om, prob = case
+    h = 1.0 / (dims[0] - 1)
+    with tuned(sweep__pair=1, sweep__pair_min_points=0, sweep2__prolong=1):
+        s = mgp.Solver(dims, levels, h, (0.1, -0.2, 0.05), bc, mgp.CONSISTENT, relax, restr, prob,
+                       flags=FLAGSETS[flags], maxiter=1, miniter=1, tolerance=0.0)
+        s.to_device()
+        o = O.Solver(dims, levels, h, (0.1, -0.2, 0.05), bc, O.CONSISTENT, relax, restr, prob)
+        o.to_device()
+        for c in range(4):
+            _it, rel, _t = s.solve(cyc, nsm, om)
+            _oit, orel, _ = o.solve(O.cycle_id(cyc), nsm, om, maxiter=1, miniter=1, tol=0.0)
+            assert rel == orel, (c, rel, orel)
+        s.to_host()
+        np.testing.assert_array_equal(s.array("u"), o.array("u"))
+        np.testing.assert_array_equal(s.array("r"), o.array("r"))
+        s.close()
+
+
+@pytest.mark.parametrize("dtype", [mgp.F64, mgp.MIXED], ids=["f64", "mixed"])
+@pytest.mark.parametrize("cycle", ["Fcycle", "Vcycle"])
+def test_prolongation_in_sweep_pass_solve_equals_separate(dtype, cycle):
+    """A solve whose corrections go into the two-sweep pass after them
+    (sweep2.prolong 2: FP64 and FP32 levels) ends with the same bits as one with a separate
+    prolongation pass (sweep2.prolong 0), sweep pairs on every level:
+    iterations, relative residual and solution (FP64 and FP32 cycles inside
+    FP64 refinement)."""
+    say("case prolongation in sweep pass, solve", dtype, cycle)
+    dims, levels = (65, 65, 33), 4
+    out = {}
+    for fuse in (0, 2):
+        with tuned(sweep__pair=1, sweep__pair_min_points=0, sweep2__prolong=fuse):
+            s = mgp.Solver(dims, levels, 1.0 / 64, (0, 0, 0), "NNNNDD", mgp.CONSISTENT, mgp.GAUSS_SEIDEL,
+                           mgp.FULL_WEIGHTING, 2, dtype=dtype, maxiter=6, miniter=6, tolerance=0.0)
+            s.to_device()
+            it, rel, _ = s.solve(cycle, 3, 1.0)
+            s.to_host()
+            out[fuse] = (it, rel, s.array("u").copy())
+            s.close()
+    assert out[1][:2] == out[0][:2]
+    np.testing.assert_array_equal(out[1][2], out[0][2])
+
+
 @pytest.mark.parametrize("dtype", [mgp.F64, mgp.MIXED], ids=["f64", "mixed"])
 @pytest.mark.parametrize("nsmooth", [2, 3])
 def test_paired_sweeps_solve_equals_single_sweep_solve(dtype, nsmooth):
