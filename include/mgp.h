@@ -603,7 +603,9 @@ int mgp_cycle_noalloc_f64(const char* cycle, const mgp_array* u, const mgp_array
  * in consistent mode the cycles add the prolongation of the coarse correction
  * inside the first two-sweep pass after it, mgp_rbgs_sweep2_prolong_*, on FP64
  * levels, default; 2: on FP32 levels too; 0: mgp_prolong_add_* and then the
- * sweeps), "residual.target_wgs",
+ * sweeps), "sweep2.pack" (consistent-mode two-sweep stages evaluate each pair
+ * of register rows once, each lane taking the row of the stage's colour: bit 0
+ * FP64, bit 1 FP32; default 3), "residual.target_wgs",
  * "residual_fw.target_wgs", "prolong.target_wgs" (waves the norm-only
  * residual, the fused residual + full-weighting and the prolongation marches
  * aim for; 0 = the value of

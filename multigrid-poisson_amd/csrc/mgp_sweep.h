@@ -21,6 +21,7 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
 extern int64_t g_sweep_pair;
 extern int64_t g_sweep2_rows;
 extern int64_t g_sweep2_prolong;
+extern int64_t g_sweep2_pack;
 // residual on a box through wave tiles (norm: max |r| over the whole shape
 // folded into *norm); MGP_ERR_ARG when the layouts do not allow it
 template <class T>

@@ -549,7 +549,7 @@ SWEEP2_SHAPES = SWEEP_SHAPES + [((3, 40, 200), "NNDDNN"), ((21, 3, 3), "NNNNNN")
                                 ((26, 17, 250), "DDDDDD")]
 # (dtype, sweep2.rows, sweep.target_wgs): several x-chunks per tile, and one
 SWEEP2_CASES = [("f64", 16, 16384), ("f64", 12, 16384), ("f64", 16, 1), ("f32", 20, 16384), ("f32", 16, 16384),
-                ("f32", 24, 1)]
+                ("f32", 24, 1), ("f64p", 16, 16384), ("f64p", 16, 1), ("f32p", 16, 16384), ("f32p", 16, 1)]
 
 
 @contextlib.contextmanager
@@ -611,6 +611,8 @@ def test_two_sweep_pass_equals_two_sweeps(shape, bc, mode, case):
     points and halo it must not write, with partial tiles in y and z, several
     and single x-chunks, and every face mix of Neumann ghosts."""
     sfx, rows, wgs = case
+    pack = {"f64p": 1, "f32p": 2}.get(sfx, 0)  # the stages' row pairs packed by colour (consistent mode)
+    sfx = sfx[:3]
     dtype = np.float64 if sfx == "f64" else np.float32
     say("case sweep2", shape, bc, mode, case)
     lv = random_level(shape, bc, mode, dtype, (shape, bc, mode, "sweep2"))
@@ -628,7 +630,7 @@ def test_two_sweep_pass_equals_two_sweeps(shape, bc, mode, case):
     other = np.random.default_rng(7).standard_normal(u0.shape)
     c = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
     d = mgp.DeviceArray(shape, halo, dtype=dtype).upload(other)
-    with tuned(sweep2__rows=rows, sweep__target_wgs=wgs):
+    with tuned(sweep2__rows=rows, sweep__target_wgs=wgs, sweep2__pack=pack):
         mgp.call("mgp_rbgs_sweep2_" + sfx, mode, c.ref(), d.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 0.93,
                  None)
     got = d.download()
@@ -707,10 +709,11 @@ PRO_SHAPES = [((9, 17, 65), "DDDDDD"), ((17, 49, 121), "NDNDND"), ((17, 41, 129)
 PRO_CHUNKS = [dict(), dict(sweep__small_min_chunk=4), dict(sweep__small_min_chunk=1024)]
 
 
+@pytest.mark.parametrize("pack", [0, 3], ids=["rows", "packed"])
 @pytest.mark.parametrize("chunks", range(len(PRO_CHUNKS)), ids=["c1", "c4", "cx"])
 @pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
 @pytest.mark.parametrize("shape,bc", PRO_SHAPES, ids=[str(i) for i in range(len(PRO_SHAPES))])
-def test_two_sweep_pass_with_prolongation_equals_prolong_add_then_pass(shape, bc, dtype, chunks):
+def test_two_sweep_pass_with_prolongation_equals_prolong_add_then_pass(shape, bc, dtype, chunks, pack):
     """mgp_rbgs_sweep2_prolong (the correction u += P(coarse) formed as each
     plane arrives, then the two sweeps) == mgp_prolong_add into a copy of u
     followed by mgp_rbgs_sweep2, bit for bit: the box points, the shape points
@@ -731,6 +734,7 @@ def test_two_sweep_pass_with_prolongation_equals_prolong_add_then_pass(shape, bc
         mgp.call("mgp_prolong_add_" + sfx, c.ref(), a.ref(), None)
         mgp.call("mgp_rbgs_sweep2_" + sfx, mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box),
                  0.93, None)
+    with tuned(sweep2__pack=pack, **PRO_CHUNKS[chunks]):
         mgp.call("mgp_rbgs_sweep2_prolong_" + sfx, mgp.CONSISTENT, got_in.ref(), got.ref(), f.ref(), ctypes.byref(F),
                  mgp.box(box), 0.93, c.ref(), None)
     want, out = b.download(), got.download()
