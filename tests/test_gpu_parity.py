@@ -812,8 +812,8 @@ def test_prolongation_in_sweep_pass_solve_equals_separate(dtype, cycle):
             s.to_host()
             out[fuse] = (it, rel, s.array("u").copy())
             s.close()
-    assert out[1][:2] == out[0][:2]
-    np.testing.assert_array_equal(out[1][2], out[0][2])
+    assert out[2][:2] == out[0][:2]
+    np.testing.assert_array_equal(out[2][2], out[0][2])
 
 
 @pytest.mark.parametrize("dtype", [mgp.F64, mgp.MIXED], ids=["f64", "mixed"])
