@@ -221,6 +221,16 @@ int mgp_rbgs_sweep2_prolong_f64(int mode, const mgp_array* u_in, const mgp_array
 int mgp_rbgs_sweep2_prolong_f32(int mode, const mgp_array* u_in, const mgp_array* u_out, const mgp_array* f,
                                 const mgp_faces* faces, const int64_t box[6], double omega,
                                 const mgp_array* coarse, void* stream);
+/* The two sweeps of u = 0: u_out = GS(GS(0)) on the box and 0 at the shape
+ * points outside it, bit for bit mgp_fill_* of 0 into a buffer followed by
+ * mgp_rbgs_sweep2_* from it, without the fill and without reading u (the
+ * first smoothing of a cycle for the correction of iterative refinement,
+ * whose initial guess is zero; no counterpart in the reference).  u_out's halo
+ * is not written.  Consistent mode only; MGP_ERR_ARG as for mgp_rbgs_sweep2_*. */
+int mgp_rbgs_sweep2_from_zero_f64(int mode, const mgp_array* u_out, const mgp_array* f, const mgp_faces* faces,
+                                  const int64_t box[6], double omega, void* stream);
+int mgp_rbgs_sweep2_from_zero_f32(int mode, const mgp_array* u_out, const mgp_array* f, const mgp_faces* faces,
+                                  const int64_t box[6], double omega, void* stream);
 /* nsweeps such sweeps in one launch, for a level small enough for one
  * workgroup's LDS ((nx+2)(ny+2)(nz+2) + nx ny nz elements <= 64 KiB;
  * MGP_ERR_ARG otherwise): the result of the last sweep goes to u_out, which may

@@ -1017,6 +1017,29 @@ static int rbgs_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const
     return march_sweep2<T>(mode, ui, uo, f, faces, b, omega, stream, coarse);
 }
 
+// two sweeps of u = 0 into u_out (consistent mode): the checks of rbgs_sweep2
+// with u_out as the layout of the (unread) input
+template <class T>
+static int rbgs_sweep2_from_zero(int mode, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
+                                 const int64_t b[6], double omega, void* stream) {
+    if (!valid_array(uo) || !valid_array(f) || !faces || !same_shape(uo, f) || mode != MGP_MODE_CONSISTENT)
+        return MGP_ERR_ARG;
+    const Box bx = box_of(b);
+    for (int d = 0; d < 3; d++)
+        if (b[2 * d] < 0 || b[2 * d + 1] > uo->shape[d]) return MGP_ERR_ARG;
+    if ((bx.x0 == 0 && faces->type[MGP_WEST] != MGP_NEUMANN) ||
+        (bx.x1 == uo->shape[0] && faces->type[MGP_EAST] != MGP_NEUMANN) ||
+        (bx.y0 == 0 && faces->type[MGP_SOUTH] != MGP_NEUMANN) ||
+        (bx.y1 == uo->shape[1] && faces->type[MGP_NORTH] != MGP_NEUMANN) ||
+        (bx.z0 == 0 && faces->type[MGP_BOTTOM] != MGP_NEUMANN) ||
+        (bx.z1 == uo->shape[2] && faces->type[MGP_TOP] != MGP_NEUMANN))
+        return MGP_ERR_ARG;
+    for (int q = 0; q < 6; q++)
+        if (faces->type[q] == MGP_NEUMANN && !faces->g[q]) return MGP_ERR_ARG;
+    if (box_empty(bx)) return MGP_ERR_ARG;
+    return march_sweep2<T>(mode, uo, uo, f, faces, b, omega, stream, nullptr, true);
+}
+
 // LDS bytes the multi-sweep kernel needs for a level, 0 when it does not fit
 static inline size_t lds_sweeps_bytes(const mgp_array* u, size_t esize) {
     const size_t b = ((size_t)(u->shape[0] + 2) * (u->shape[1] + 2) * (u->shape[2] + 2) +
@@ -1571,6 +1594,14 @@ int mgp_rbgs_sweep2_f64(int m, const mgp_array* ui, const mgp_array* uo, const m
 int mgp_rbgs_sweep2_f32(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
                         const int64_t b[6], double w, void* s) {
     return rbgs_sweep2<float>(m, ui, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep2_from_zero_f64(int m, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                                  const int64_t b[6], double w, void* s) {
+    return rbgs_sweep2_from_zero<double>(m, uo, f, F, b, w, s);
+}
+int mgp_rbgs_sweep2_from_zero_f32(int m, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,
+                                  const int64_t b[6], double w, void* s) {
+    return rbgs_sweep2_from_zero<float>(m, uo, f, F, b, w, s);
 }
 int mgp_rbgs_sweep2_prolong_f64(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,
                                 const mgp_faces* F, const int64_t b[6], double w, const mgp_array* c, void* s) {

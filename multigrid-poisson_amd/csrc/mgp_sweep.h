@@ -14,10 +14,13 @@ int march_sweep(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_ar
 // two sweeps in one pass (ui -> uo, ui != uo); MGP_ERR_ARG when the shape
 // does not allow it (fewer than 3 planes, or a plane of 1 GiB or more).  With
 // coarse (2:1 nested, checked by the caller; consistent mode only): the two
-// sweeps of ui + P(coarse), the points outside the box receiving ui + P(coarse)
+// sweeps of ui + P(coarse), the points outside the box receiving ui +
+// P(coarse).  With zero_in (consistent mode only): the two sweeps of u = 0, ui
+// giving the layout only (not read), the points outside the box receiving 0
 template <class T>
 int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
-                 const int64_t box[6], double omega, void* stream, const mgp_array* coarse = nullptr);
+                 const int64_t box[6], double omega, void* stream, const mgp_array* coarse = nullptr,
+                 bool zero_in = false);
 extern int64_t g_sweep_pair;
 extern int64_t g_sweep2_rows;
 extern int64_t g_sweep2_prolong;

@@ -116,8 +116,9 @@ inline int complement_blocks(const View<T>& u, const int64_t b[6], int threads) 
 // points [blk * blockDim * COPY_PER_THREAD, ...) of the complement, in the
 // order: x outside the box (whole planes), then y outside (box planes), then
 // z outside (box rows); consecutive threads take consecutive z where it runs.
-// PRO: ui + P(coarse) instead of ui (the pass that adds the prolongation)
-template <class T, bool PRO = false>
+// PRO: ui + P(coarse) instead of ui (the pass that adds the prolongation);
+// ZERO: 0 (the pass whose input is zero and not read)
+template <class T, bool PRO = false, bool ZERO = false>
 __device__ __forceinline__ void copy_complement(const MarchArgs<T>& A, int blk) {
     const int64_t n0 = A.ui.n0, n1 = A.ui.n1, n2 = A.ui.n2;
     const int64_t BX = A.bx1 - A.bx0, BY = A.by1 - A.by0;
@@ -143,7 +144,9 @@ __device__ __forceinline__ void copy_complement(const MarchArgs<T>& A, int blk) 
             y = A.by0 + r / zo;
             z = c < A.bz0 ? c : A.bz1 + (c - A.bz0);
         }
-        if (PRO)
+        if (ZERO)
+            A.uo.at(x, y, z) = (T)0;
+        else if (PRO)
             A.uo.at(x, y, z) = A.ui.at(x, y, z) + trilinear<T>(A.c, x, y, z);
         else
             A.uo.at(x, y, z) = A.ui.at(x, y, z);
@@ -765,7 +768,8 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
 // afresh before every stage).
 constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
 
-template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0, bool PACK = false>
+template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0, bool PACK = false,
+          bool ZIN = false>
 __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     constexpr int RO = NR - 8;  // rows written per tile
     const int tz = tl % A.ztiles;
@@ -809,6 +813,11 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     const bool z_ghosts = (z0 - 4 < 0 && gB) || (z0 + 59 >= n2 && gT);
 
     auto load_plane = [&](int64_t j, T (&P)[NR]) {
+        if constexpr (ZIN) {  // an input of zeros, not read
+#pragma unroll
+            for (int t = 0; t < NR; t++) P[t] = (T)0;
+            return;
+        }
         const __amdgpu_buffer_rsrc_t r = plane_rsrc(A.ui.p + min(max(j, (int64_t)0), n0 - 1) * s0, plane_bytes);
 #pragma unroll
         for (int t = 0; t < NR; t++) P[t] = bload<T>(r, zoff, roff[t]);
@@ -1039,6 +1048,23 @@ __global__ void __launch_bounds__(64) k_rbgs2_wave(MarchArgs<T> A) {
     if (tl >= A.ntiles) return;
     if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MODE, NR, true, false, 0, 0, PACK>(A, tl);
     else wave_march2<T, MODE, NR, false, false, 0, 0, PACK>(A, tl);
+}
+
+// the two sweeps of u = 0 (consistent mode): u_in is not read, the shape
+// points outside the box are written 0
+template <class T, int NR, bool PACK>
+__global__ void __launch_bounds__(64) k_rbgs2z_wave(MarchArgs<T> A) {
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement<T, false, true>(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    if (sweep2_edge_tile<T, NR>(A, tl))
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, true, false, 0, 0, PACK, true>(A, tl);
+    else
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, false, false, 0, 0, PACK, true>(A, tl);
 }
 
 // the two sweeps after u_in + P(coarse) (consistent mode); PY, PZ: parities of
@@ -2376,7 +2402,7 @@ int64_t g_sweep2_prolong = 1;  // the cycles add the prolongation inside the fir
 
 template <class T>
 int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_array* f, const mgp_faces* faces,
-                 const int64_t b[6], double omega, void* stream, const mgp_array* coarse) {
+                 const int64_t b[6], double omega, void* stream, const mgp_array* coarse, bool zero_in) {
     MarchArgs<T> A;
     A.ui = view_of<T>(*ui);
     A.uo = view_of<T>(*uo);
@@ -2387,8 +2413,9 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     if (A.ui.n0 < 3 || plane_bytes >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
     if (coarse) {
         const int64_t cplane = ((A.c.n1 - 1) * A.c.s1 + A.c.n2) * (int64_t)sizeof(T);
-        if (mode != MGP_MODE_CONSISTENT || cplane >= (int64_t)BUF_OOB) return MGP_ERR_ARG;
+        if (mode != MGP_MODE_CONSISTENT || cplane >= (int64_t)BUF_OOB || zero_in) return MGP_ERR_ARG;
     }
+    if (zero_in && mode != MGP_MODE_CONSISTENT) return MGP_ERR_ARG;
     for (int q = 0; q < 6; q++) A.g[q] = faces->type[q] == MGP_NEUMANN ? static_cast<const T*>(faces->g[q]) : nullptr;
     A.two_h = (T)(2.0 * faces->h);  // include/neumann.h:108
     A.bx0 = b[0], A.bx1 = b[1], A.by0 = b[2], A.by1 = b[3], A.bz0 = b[4], A.bz1 = b[5];
@@ -2400,7 +2427,7 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     int rows = (int)g_sweep2_rows;
     if (sizeof(T) == 8 && rows != 12 && rows != 16) rows = 16;
     if (sizeof(T) == 4 && rows != 16 && rows != 20 && rows != 24) rows = 16;
-    if (coarse) rows = 16;  // the one tile height with the prolongation built in
+    if (coarse || zero_in) rows = 16;  // the one tile height with the prolongation or zero input built in
     A.ytiles = (int)((b[3] - b[2] + rows - 9) / (rows - 8));
     A.ztiles = (int)((b[5] - b[4] + WZ2 - 1) / WZ2);
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
@@ -2436,6 +2463,12 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
         const hipError_t e = hipGetLastError();
         return e == hipSuccess ? MGP_OK : (int)e;
     }
+    if (zero_in) {
+        if (pack) hipLaunchKernelGGL((k_rbgs2z_wave<T, 16, true>), g, dim3(64), 0, st, A);
+        else hipLaunchKernelGGL((k_rbgs2z_wave<T, 16, false>), g, dim3(64), 0, st, A);
+        const hipError_t e = hipGetLastError();
+        return e == hipSuccess ? MGP_OK : (int)e;
+    }
     if (pack) {
         hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_CONSISTENT, 16, true>), g, dim3(64), 0, st, A);
         const hipError_t e = hipGetLastError();
@@ -2455,8 +2488,8 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     return e == hipSuccess ? MGP_OK : (int)e;
 }
 template int march_sweep2<double>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
-                                  const int64_t[6], double, void*, const mgp_array*);
+                                  const int64_t[6], double, void*, const mgp_array*, bool);
 template int march_sweep2<float>(int, const mgp_array*, const mgp_array*, const mgp_array*, const mgp_faces*,
-                                 const int64_t[6], double, void*, const mgp_array*);
+                                 const int64_t[6], double, void*, const mgp_array*, bool);
 
 }  // namespace mgp

@@ -137,6 +137,7 @@ for sfx in ("f64", "f32"):
         "mgp_rbgs_sweep_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, vp],
         "mgp_rbgs_sweep2_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, vp],
         "mgp_rbgs_sweep2_prolong_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, P(Array), vp],
+        "mgp_rbgs_sweep2_from_zero_" + sfx: [i32, P(Array), P(Array), P(Faces), BOX, d, vp],
         "mgp_rbgs_sweeps_" + sfx: [i32, P(Array), P(Array), P(Array), P(Faces), BOX, d, i64, vp],
         "mgp_residual_restrict_fw_" + sfx: [i32, P(Array), P(Array), P(Array), P(Array), BOX, d, vp],
         "mgp_residual_norm_" + sfx: [i32, P(Array), P(Array), P(Array), BOX, d, vp, vp],

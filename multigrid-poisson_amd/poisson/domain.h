@@ -81,6 +81,16 @@ public:
     CoarseLevels<T>* below = nullptr;
     // addition: the last level of its solver's hierarchy
     bool coarsest = false;
+    // addition: u is zero in value though its buffer was not written (the
+    // correction of an iterative-refinement step before its cycle); the first
+    // smoothing takes it as zero without reading it (mgp_rbgs_sweep2_from_zero_*),
+    // and any other use of u first writes the zeros (materialize_zero)
+    bool u_zero = false;
+    void materialize_zero() {
+        if (!u_zero) return;
+        u->init_zero();
+        u_zero = false;
+    }
 
     Domain() : halo(), is_initialized(false), requires_duplicate(true), settings() {}
     Domain(Settings& _settings, BoundaryCondition& BC, bool requires_duplicate, int_t device[3], int_t device_shape[3]);
@@ -133,6 +143,7 @@ public:
     // u (and uprev) = 0 with the boundary data written again, on the device;
     // the state to_device() leaves for a fresh solve with unchanged f and faces
     void reset_solution() {
+        u_zero = false;
         u->init_zero();
         write_bc_to(*u);
         if (requires_duplicate) {
@@ -232,6 +243,7 @@ void Domain<T>::init_zero() {
 // (and uprev); the out-of-place sweep buffer starts as a copy of u
 template <class T>
 void Domain<T>::to_device() {
+    u_zero = false;
     this->u->to_device();
     if (requires_duplicate) this->uprev->to_device();
     this->f->to_device();
@@ -251,6 +263,7 @@ void Domain<T>::to_device() {
 
 template <class T>
 void Domain<T>::to_device_zero() {
+    u_zero = false;
     DeviceArray<T>* arrays[] = {u, requires_duplicate ? uprev : nullptr, f, r, &north->arr, &south->arr,
                                 &east->arr, &west->arr, &top->arr, &bottom->arr};
     for (DeviceArray<T>* a : arrays) {

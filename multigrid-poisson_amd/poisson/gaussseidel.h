@@ -41,6 +41,8 @@ public:
     }
 
     void relax_level(Domain<T>& d, T omega, int_t nsmooth) override {
+        if (d.u_zero && from_zero(d, omega, nsmooth)) return;
+        d.materialize_zero();
         if (!d.u_alt || !d.settings.fused_sweeps || !checkerboard(d) || !alternate || nsmooth < 1) {
             Relaxation<T>::relax_level(d, omega, nsmooth);
             return;
@@ -104,8 +106,36 @@ public:
     // least "sweep.pair_min_points" points, "sweep2.prolong" 1 (FP64 levels) or
     // 2 (FP32 levels too: there the separate passes are faster, 2.47 + 1.01 ms
     // against 4.00 ms at 1025 x 1025 x 513; FP64 4.91 + 2.04 against 5.81 ms)
+    // u zero in value, its buffer unwritten (Domain::u_zero): where the first
+    // pair of sweeps runs through the two-sweep pass, that pass starts from
+    // zeros without reading u (mgp_rbgs_sweep2_from_zero_*); false where it does
+    // not apply (the caller then writes the zeros and smooths as usual)
+    bool from_zero(Domain<T>& d, T omega, int_t nsmooth) {
+        int64_t pair = 1, pair_min = 0;
+        if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
+        if (mgp_tuning_get("sweep.pair_min_points", &pair_min) != MGP_OK) pair_min = 0;
+        if (!(pair && d.u_alt && d.settings.fused_sweeps && d.settings.mode == CONSISTENT && alternate &&
+              nsmooth >= 2 && d.u->shape[0] * d.u->shape[1] * d.u->shape[2] >= pair_min))
+            return false;
+        int64_t b[6];
+        d.box(b);
+        const mgp_faces F = d.faces();
+        const mgp_array f = d.f->desc();
+        mgp_array uo = d.u->desc();
+        uo.data = d.u_alt;
+        const int rc = mgpx::Ops<T>::rbgs_sweep2_from_zero(d.settings.mode, &uo, &f, &F, b, (double)omega,
+                                                           mgpx::stream());
+        if (rc == MGP_ERR_ARG) return false;
+        mgpx::check(rc, "GaussSeidel sweep pair from zero");
+        d.swap_u_alt();
+        d.u_zero = false;
+        if (nsmooth > 2) relax_level(d, omega, nsmooth - 2);
+        return true;
+    }
+
     void relax_level_prolonged(Domain<T>& d, Domain<T>& coarse, Prolongation<T>& prolongation, T omega,
                                int_t nsmooth) override {
+        d.materialize_zero();
         int64_t pair = 1, pair_min = 0, fuse = 1;
         if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
         if (mgp_tuning_get("sweep.pair_min_points", &pair_min) != MGP_OK) pair_min = 0;

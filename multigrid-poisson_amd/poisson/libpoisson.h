@@ -60,6 +60,7 @@ private:
         DeviceArray<T>* uprev;
         T* devptr;
         T* alt;
+        bool zero;
     };
     struct State {
         std::vector<LevelState> levels;
@@ -71,6 +72,7 @@ private:
                 if (a.u != b.u) return a.u < b.u;
                 if (a.devptr != b.devptr) return a.devptr < b.devptr;
                 if (a.alt != b.alt) return a.alt < b.alt;
+                if (a.zero != b.zero) return a.zero < b.zero;
             }
             return false;
         }
@@ -281,7 +283,7 @@ typename PoissonSolver<T, R, P, S>::State PoissonSolver<T, R, P, S>::capture_sta
     s.relax_state = relaxation.mgp_state();
     for (uint_t l = 0; l < settings.levels; l++) {
         Domain<T>& d = *domains[0][l];
-        s.levels.push_back(LevelState{d.u, d.uprev, d.u->devptr, d.u_alt});
+        s.levels.push_back(LevelState{d.u, d.uprev, d.u->devptr, d.u_alt, d.u_zero});
     }
     return s;
 }
@@ -295,6 +297,7 @@ void PoissonSolver<T, R, P, S>::apply_state(const State& s) {
         d.uprev = s.levels[l].uprev;
         d.u->devptr = s.levels[l].devptr;
         d.u_alt = s.levels[l].alt;
+        d.u_zero = s.levels[l].zero;
     }
 }
 

@@ -108,6 +108,10 @@ template <class T> struct Ops;
                                        void* s) {                                                                \
             return mgp_rbgs_sweep2_prolong_##SFX(m, ui, uo, f, F, b, w, c, s);                                   \
         }                                                                                                        \
+        static int rbgs_sweep2_from_zero(int m, const mgp_array* uo, const mgp_array* f, const mgp_faces* F,     \
+                                         const int64_t b[6], double w, void* s) {                                \
+            return mgp_rbgs_sweep2_from_zero_##SFX(m, uo, f, F, b, w, s);                                        \
+        }                                                                                                        \
         static int rbgs_sweeps(int m, const mgp_array* ui, const mgp_array* uo, const mgp_array* f,              \
                                const mgp_faces* F, const int64_t b[6], double w, int64_t n, void* s) {           \
             return mgp_rbgs_sweeps_##SFX(m, ui, uo, f, F, b, w, n, s);                                            \

@@ -201,8 +201,11 @@ public:
         Domain<float>& d32 = inner->get_domain(0, 0);
         const mgp_array r64 = d64.r->desc(), f32 = d32.f->desc(), u64 = d64.u->desc();
         if (!f32_ready) mgpx::check(mgp_cast_f64_f32(&r64, &f32, st), "refinement: cast");
-        d32.u->init_zero();
+        // e = 0: taken as zero by the cycle's first smoothing, which then does
+        // not read it (written as zeros wherever it would be read instead)
+        d32.u_zero = true;
         inner->cycle(fcycle ? "Fcycle" : "Vcycle", nsmooth, (float)omega);
+        d32.materialize_zero();  // (consumed by every cycle; kept for any path that does not)
         const mgp_array e32 = d32.u->desc();
         mgpx::check(mgp_zero_scalars(dev_norm, 1, st), "refinement: norm");
         if (lo[0]) {

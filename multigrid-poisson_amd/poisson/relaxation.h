@@ -30,6 +30,7 @@ public:
     // one level: prolong_add into d.u (d.r as the scratch), then relax_level
     virtual void relax_level_prolonged(Domain<T>& d, Domain<T>& coarse, Prolongation<T>& prolongation, T omega,
                                        int_t nsmooth) {
+        d.materialize_zero();
         prolongation.prolong_add(*coarse.u, *d.u, *d.r);
         relax_level(d, omega, nsmooth);
     }
@@ -49,13 +50,16 @@ void Relaxation<T>::relax_prolonged(Domain<T>*** domains, Prolongation<T>& prolo
         this->relax_level_prolonged(*domains[0][level], *domains[0][level + 1], prolongation, omega, nsmooth);
         return;
     }
-    for (int_t g = 0; g < num_devices; g++)
+    for (int_t g = 0; g < num_devices; g++) {
+        domains[g][level]->materialize_zero();
         prolongation.prolong_add(*domains[g][level + 1]->u, *domains[g][level]->u, *domains[g][level]->r);
+    }
     relax(domains, omega, nsmooth, level, num_devices);
 }
 
 template <class T>
 void Relaxation<T>::relax_level(Domain<T>& domain, T omega, int_t nsmooth) {
+    domain.materialize_zero();
     const bool dup = requires_duplicate_solution();
     for (int_t s = 0; s < smoothing_multiplier() * nsmooth; s++) {
         if (dup) domain.swap_u();

@@ -19,6 +19,7 @@ void descend(Domain<T>*** domains, Restriction<T>& restriction, uint_t level, ui
     for (uint_t g = 0; g < num_devices; g++) {
         Domain<T>& d = *domains[g][level];
         Domain<T>& c = *domains[g][level + 1];
+        d.materialize_zero();
         // consistent mode with fused kernels: residual and full weighting in
         // one pass (the fine r is not needed afterwards: the prolongation is
         // fused with the correction)
@@ -49,6 +50,7 @@ void descend(Domain<T>*** domains, Restriction<T>& restriction, uint_t level, ui
 // on u; false where the level does not fit one workgroup (smoothing then)
 template <class T>
 bool coarse_cg_solve(Domain<T>& d) {
+    d.materialize_zero();
     int64_t b[6];
     d.box(b);
     const mgp_array u = d.u->desc(), f = d.f->desc();

@@ -747,6 +747,36 @@ def test_two_sweep_pass_with_prolongation_equals_prolong_add_then_pass(shape, bc
     np.testing.assert_array_equal(c.download(), c0.astype(dtype))
 
 
+@pytest.mark.parametrize("pack", [0, 3], ids=["rows", "packed"])
+@pytest.mark.parametrize("dtype", [np.float64, np.float32], ids=["f64", "f32"])
+@pytest.mark.parametrize("shape,bc", PRO_SHAPES, ids=[str(i) for i in range(len(PRO_SHAPES))])
+def test_two_sweep_pass_from_zero_equals_fill_then_pass(shape, bc, dtype, pack):
+    """mgp_rbgs_sweep2_from_zero (two sweeps of u = 0, u never read) == a fill
+    of 0 followed by mgp_rbgs_sweep2, bit for bit on the shape; the output
+    halo untouched.  Homogeneous and inhomogeneous Neumann data."""
+    sfx = "f64" if dtype == np.float64 else "f32"
+    say("case sweep2 from zero", shape, bc, sfx, pack)
+    halo, u0, f, faces, F, box = random_level(shape, bc, mgp.CONSISTENT, dtype, (shape, bc, "sweep2z"))
+    a = mgp.DeviceArray(shape, halo, dtype=dtype).upload(np.zeros_like(u0))
+    b = mgp.DeviceArray(shape, halo, dtype=dtype).upload(u0)
+    other = np.random.default_rng(11).standard_normal(u0.shape)
+    got = mgp.DeviceArray(shape, halo, dtype=dtype).upload(other)
+    with tuned(sweep2__pack=pack):
+        mgp.call("mgp_rbgs_sweep2_" + sfx, mgp.CONSISTENT, a.ref(), b.ref(), f.ref(), ctypes.byref(F), mgp.box(box),
+                 0.93, None)
+        mgp.call("mgp_rbgs_sweep2_from_zero_" + sfx, mgp.CONSISTENT, got.ref(), f.ref(), ctypes.byref(F),
+                 mgp.box(box), 0.93, None)
+    want, out = b.download(), got.download()
+    inner = shape_slices(shape, halo)
+    np.testing.assert_array_equal(out[inner], want[inner])
+    outside = np.ones(out.shape, bool)
+    outside[inner] = False
+    np.testing.assert_array_equal(out[outside], other.astype(dtype)[outside])
+    L = mgp.lib()
+    assert L.mgp_rbgs_sweep2_from_zero_f64(mgp.REFERENCE, got.ref(), f.ref(), ctypes.byref(F), mgp.box(box), 1.0,
+                                           None) == mgp.ERR_ARG
+
+
 def test_two_sweep_pass_with_prolongation_rejects_bad_coarse_levels():
     """Reference mode, a coarse level that is not 2:1 nested, and a coarse
     level that is the input or output buffer are refused (MGP_ERR_ARG)."""
