@@ -758,8 +758,8 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
 // mgp_prolong_add_* (x-, then y-, then z-midpoints, each 0.5 * (left + right),
 // include/trilinearinterpolation.h:50-133), so the values are the same.  Each
 // lane keeps one coarse column q at rows cy0 .. cy0 + NR/2 of the two coarse
-// planes under the fine plane j (Ca: j >> 1, Cb: (j >> 1) + 1); an odd z takes
-// its second column from the neighbouring lane.  PY is the parity of y at
+// planes under the fine plane j (j >> 1 and (j >> 1) + 1, in LDS); an odd z
+// takes its second column from the neighbouring lane.  PY is the parity of y at
 // register row 0 and PZ that of z at lane 0 (the launch fixes both: tiles step
 // by even counts): PZ = 0 has each lane read column (z + 1) >> 1, so an odd z
 // finds column z >> 1 on the lane before; PZ = 1 has it read z >> 1 and find
@@ -770,7 +770,7 @@ constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
 
 template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0, bool PACK = false,
           bool ZIN = false>
-__device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
+__device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl = nullptr) {
     constexpr int RO = NR - 8;  // rows written per tile
     const int tz = tl % A.ztiles;
     const int ty = (tl / A.ztiles) % A.ytiles;
@@ -833,7 +833,13 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     // (edge tiles only; no update reads what they feed) fall outside the
     // plane's record range and read 0
     constexpr int CR = NR / 2 + 1;
-    T Ca[CR], Cb[CR];
+    // The two coarse planes live in the wave's LDS (cl, slots of CR x 64
+    // values: plane j >> 1 in slot cs, (j >> 1) + 1 in slot cs ^ 1); a new
+    // plane is loaded into registers (Ct) at the start of an even fine plane's
+    // step and stored to its slot at the end of the step, when it has arrived
+    T Ct[CR];
+    int cs = 0;
+    bool ct_pending = false;
     int32_t cvoff = 0, crow = 0;
     uint32_t cplane = 0;
     if constexpr (PRO) {
@@ -844,18 +850,23 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
         cvoff = (int32_t)(cy0 * crow + q * (int64_t)sizeof(T));
         cplane = (uint32_t)(((c1 - 1) * A.c.s1 + c2) * (int64_t)sizeof(T));
     }
+    auto cl_at = [&](int slot, int b) -> T& { return cl[(slot * CR + b) * 64 + lane]; };
     auto load_coarse = [&](int64_t a, T (&V)[CR]) {
         const __amdgpu_buffer_rsrc_t r =
             plane_rsrc(A.c.p + min(max(a, (int64_t)0), A.c.n0 - 1) * A.c.s0, cplane);
 #pragma unroll
         for (int b = 0; b < CR; b++) V[b] = bload<T>(r, (uint32_t)(cvoff + b * crow), 0);
     };
-    // Ca, Cb for fine plane j, given them for plane j - 1
+    auto store_coarse = [&](int slot, const T (&V)[CR]) {
+#pragma unroll
+        for (int b = 0; b < CR; b++) cl_at(slot, b) = V[b];
+    };
+    // the slots for fine plane j, given them for plane j - 1
     auto next_coarse = [&](int64_t j) {
         if (j & 1) return;
-#pragma unroll
-        for (int b = 0; b < CR; b++) Ca[b] = Cb[b];
-        load_coarse((j >> 1) + 1, Cb);
+        cs ^= 1;  // plane (j >> 1) was the second plane of j - 1
+        load_coarse((j >> 1) + 1, Ct);
+        ct_pending = true;
     };
     // P[t] += P(coarse) from the x-stage values X (coarse rows cy0 + b): the
     // y-midpoint on odd rows, then the z-midpoint 0.5 * (Y + o) with o the
@@ -873,14 +884,19 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     };
     // P[t] += P(coarse) at (j, y0 - 4 + t, z)
     auto add_prolong = [&](int64_t j, T (&P)[NR]) {
-        if (j & 1) {
-            T X[CR];
-#pragma unroll
-            for (int b = 0; b < CR; b++) X[b] = half_sum<T>(Ca[b], Cb[b]);
-            add_rows(X, P);
-        } else {
-            add_rows(Ca, P);
+        if (ct_pending) {
+            store_coarse(cs ^ 1, Ct);
+            ct_pending = false;
         }
+        T X[CR];
+        if (j & 1) {
+#pragma unroll
+            for (int b = 0; b < CR; b++) X[b] = half_sum<T>(cl_at(cs, b), cl_at(cs ^ 1, b));
+        } else {
+#pragma unroll
+            for (int b = 0; b < CR; b++) X[b] = cl_at(cs, b);
+        }
+        add_rows(X, P);
     };
 
     // stage S at plane p (S even: red, odd: black) with west / centre / east
@@ -982,8 +998,10 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl) {
     load_f(i0, F0);
     if constexpr (PRO) {
         const int64_t j0 = i0 - 1;
-        load_coarse(j0 >> 1, Ca);
-        load_coarse((j0 >> 1) + 1, Cb);
+        load_coarse(j0 >> 1, Ct);
+        store_coarse(0, Ct);
+        load_coarse((j0 >> 1) + 1, Ct);
+        store_coarse(1, Ct);
         add_prolong(j0, Pm1);
         next_coarse(j0 + 1);
         add_prolong(j0 + 1, P0);
@@ -1078,8 +1096,9 @@ __global__ void __launch_bounds__(64) k_rbgs2p_wave(MarchArgs<T> A) {
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (tl >= A.ntiles) return;
-    if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MGP_MODE_CONSISTENT, NR, true, true, PY, PZ, PACK>(A, tl);
-    else wave_march2<T, MGP_MODE_CONSISTENT, NR, false, true, PY, PZ, PACK>(A, tl);
+    __shared__ T cl[2 * (NR / 2 + 1) * 64];  // the wave's two coarse planes
+    if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MGP_MODE_CONSISTENT, NR, true, true, PY, PZ, PACK>(A, tl, cl);
+    else wave_march2<T, MGP_MODE_CONSISTENT, NR, false, true, PY, PZ, PACK>(A, tl, cl);
 }
 
 

@@ -104,8 +104,8 @@ public:
     // (mgp_rbgs_sweep2_prolong_*) where relax_level would run that pair through
     // the two-sweep pass: consistent mode, trilinear prolongation, a level of at
     // least "sweep.pair_min_points" points, "sweep2.prolong" 1 (FP64 levels) or
-    // 2 (FP32 levels too: there the separate passes are faster, 2.47 + 1.01 ms
-    // against 4.00 ms at 1025 x 1025 x 513; FP64 4.91 + 2.04 against 5.81 ms)
+    // 2 (FP32 levels too: there the separate passes are faster, 2.30 + 1.01 ms
+    // against 3.40 ms at 1025 x 1025 x 513; FP64 4.54 + 2.05 against 5.18 ms)
     // u zero in value, its buffer unwritten (Domain::u_zero): where the first
     // pair of sweeps runs through the two-sweep pass, that pass starts from
     // zeros without reading u (mgp_rbgs_sweep2_from_zero_*); false where it does
