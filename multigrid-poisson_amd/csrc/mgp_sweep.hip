@@ -769,8 +769,13 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
 constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
 
 template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0, bool PACK = false,
-          bool ZIN = false>
-__device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl = nullptr) {
+          bool ZIN = false, bool FLDS = false>
+__device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl = nullptr, T* fr = nullptr) {
+    // FLDS (with PACK): the f planes in the wave's LDS (fr: a ring of 4 planes x
+    // NR-2 rows x 64 lanes, plane q in slot q & 3) instead of registers, each
+    // lane reading the row its packed pair takes.  1025 x 1025 x 513 with the
+    // correction inside: FP32 2.43 ms against 3.40 ms; FP64 5.18 ms either way;
+    // the plain passes are 1 % faster with f in registers
     constexpr int RO = NR - 8;  // rows written per tile
     const int tz = tl % A.ztiles;
     const int ty = (tl / A.ztiles) % A.ytiles;
@@ -902,6 +907,7 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl
     // stage S at plane p (S even: red, odd: black) with west / centre / east
     // planes W, C, E and f of plane p; rows S+1 .. NR-2-S are updated, rows
     // S .. NR-1-S read
+    auto fr_at = [&](int64_t q, int row) -> T& { return fr[(((int)q & 3) * (NR - 2) + row - 1) * 64 + lane]; };
     auto stage = [&](auto Sc, int64_t p, T (&W)[NR], T (&C)[NR], T (&E)[NR], const T (&F)[NR]) {
         constexpr int S = decltype(Sc)::value;
         // x-ghost planes from the plane two apart (face layout (1, ny, nz))
@@ -969,7 +975,8 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl
                 const T x = m ? C[t + 1] : C[t];
                 const T v = update<T, MODE>(c, m ? W[t] : W[t + 1], m ? E[t] : E[t + 1], m ? C[t - 1] : C[t],
                                             m ? C[t + 1] : C[t + 2], lane_prev(x), lane_next(x),
-                                            m ? F[t] : F[t + 1], A.one_omega, A.omega_sixth, A.hsq);
+                                            FLDS ? fr_at(p, m ? t : t + 1) : (m ? F[t] : F[t + 1]), A.one_omega,
+                                            A.omega_sixth, A.hsq);
                 const bool ok0 = lane_ok && (!EDGE || ybox[t]), ok1 = lane_ok && (!EDGE || ybox[t + 1]);
                 C[t] = m && ok0 ? v : C[t];
                 C[t + 1] = !m && ok1 ? v : C[t + 1];
@@ -996,6 +1003,10 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl
     load_plane(i0, P0);
     load_plane(i0 + 1, Pp1);
     load_f(i0, F0);
+    if constexpr (FLDS) {
+#pragma unroll
+        for (int t = 1; t <= NR - 2; t++) fr_at(i0, t) = F0[t];
+    }
     if constexpr (PRO) {
         const int64_t j0 = i0 - 1;
         load_coarse(j0 >> 1, Ct);
@@ -1028,6 +1039,10 @@ __device__ __forceinline__ void wave_march2(const MarchArgs<T>& A, int tl, T* cl
             for (int t = 4; t <= NR - 5; t++) bstore(Pm3[t], ro, !EDGE || ybox[t] ? zoff_st : BUF_OOB, roff[t]);
         }
         if constexpr (PRO) add_prolong(i + 2, PL);
+        if constexpr (FLDS) {  // f of plane i+1 into the slot plane i-3 used
+#pragma unroll
+            for (int t = 1; t <= NR - 2; t++) fr_at(i + 1, t) = FL[t];
+        }
 #pragma unroll
         for (int t = 0; t < NR; t++) {
             Pm4[t] = Pm3[t];
@@ -1096,9 +1111,12 @@ __global__ void __launch_bounds__(64) k_rbgs2p_wave(MarchArgs<T> A) {
     const int per_xcd = A.ntiles_pad / 8;
     const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
     if (tl >= A.ntiles) return;
-    __shared__ T cl[2 * (NR / 2 + 1) * 64];  // the wave's two coarse planes
-    if (sweep2_edge_tile<T, NR>(A, tl)) wave_march2<T, MGP_MODE_CONSISTENT, NR, true, true, PY, PZ, PACK>(A, tl, cl);
-    else wave_march2<T, MGP_MODE_CONSISTENT, NR, false, true, PY, PZ, PACK>(A, tl, cl);
+    __shared__ T cl[2 * (NR / 2 + 1) * 64];          // the wave's two coarse planes
+    __shared__ T fr[PACK ? 4 * (NR - 2) * 64 : 1];  // its f planes (packed stages)
+    if (sweep2_edge_tile<T, NR>(A, tl))
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, true, true, PY, PZ, PACK, false, PACK>(A, tl, cl, fr);
+    else
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, false, true, PY, PZ, PACK, false, PACK>(A, tl, cl, fr);
 }
 
 
@@ -2416,7 +2434,7 @@ int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, 
 // FP64, bit 1 FP32).  1025 x 1025 x 513: FP64 4.53 ms against 4.92 ms per pass,
 // with the correction inside 5.71 against 5.80 ms; FP32 2.29 against 2.48 ms
 int64_t g_sweep2_pack = 3;
-int64_t g_sweep2_prolong = 1;  // the cycles add the prolongation inside the first post-smoothing pass
+int64_t g_sweep2_prolong = 2;  // the cycles add the prolongation inside the first post-smoothing pass
                                // (1: FP64 levels, 2: FP32 levels too)
 
 template <class T>

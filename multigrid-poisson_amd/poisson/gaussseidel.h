@@ -100,12 +100,6 @@ public:
         }
     }
 
-    // the correction and the first pair of sweeps in one pass
-    // (mgp_rbgs_sweep2_prolong_*) where relax_level would run that pair through
-    // the two-sweep pass: consistent mode, trilinear prolongation, a level of at
-    // least "sweep.pair_min_points" points, "sweep2.prolong" 1 (FP64 levels) or
-    // 2 (FP32 levels too: there the separate passes are faster, 2.30 + 1.01 ms
-    // against 3.40 ms at 1025 x 1025 x 513; FP64 4.54 + 2.05 against 5.18 ms)
     // u zero in value, its buffer unwritten (Domain::u_zero): where the first
     // pair of sweeps runs through the two-sweep pass, that pass starts from
     // zeros without reading u (mgp_rbgs_sweep2_from_zero_*); false where it does
@@ -133,13 +127,20 @@ public:
         return true;
     }
 
+    // the correction and the first pair of sweeps in one pass
+    // (mgp_rbgs_sweep2_prolong_*) where relax_level would run that pair through
+    // the two-sweep pass: consistent mode, trilinear prolongation, a level of at
+    // least "sweep.pair_min_points" points, "sweep2.prolong" 2 (FP64 and FP32
+    // levels, default) or 1 (FP64 levels only).  1025 x 1025 x 513: FP64 5.18 ms
+    // against 4.54 + 2.05 ms for the separate passes, FP32 2.43 against
+    // 2.30 + 1.01 ms
     void relax_level_prolonged(Domain<T>& d, Domain<T>& coarse, Prolongation<T>& prolongation, T omega,
                                int_t nsmooth) override {
         d.materialize_zero();
-        int64_t pair = 1, pair_min = 0, fuse = 1;
+        int64_t pair = 1, pair_min = 0, fuse = 2;
         if (mgp_tuning_get("sweep.pair", &pair) != MGP_OK) pair = 1;
         if (mgp_tuning_get("sweep.pair_min_points", &pair_min) != MGP_OK) pair_min = 0;
-        if (mgp_tuning_get("sweep2.prolong", &fuse) != MGP_OK) fuse = 1;
+        if (mgp_tuning_get("sweep2.prolong", &fuse) != MGP_OK) fuse = 2;
         if (!std::is_same<T, double>::value && fuse < 2) fuse = 0;
         const bool applies = fuse && pair && d.u_alt && d.settings.fused_sweeps && d.settings.mode == CONSISTENT &&
                              alternate && nsmooth >= 2 && prolongation.mgp_trilinear() &&
