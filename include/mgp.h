@@ -609,7 +609,7 @@ int mgp_cycle_noalloc_f64(const char* cycle, const mgp_array* u, const mgp_array
  * (register rows per two-sweep
  * wave tile: 12 or 16 for FP64, 16, 20 or 24 for FP32; 0 = 16, default),
  * "sweep2.chunks_per_tile" (least x chunks per two-sweep wave tile, made odd,
- * and at least enough for about 6800 waves; default 15), "sweep2.prolong" (2:
+ * and at least enough for "sweep2.target_waves" waves; defaults 15 and 6800), "sweep2.prolong" (2:
  * in consistent mode the cycles add the prolongation of the coarse correction
  * inside the first two-sweep pass after it, mgp_rbgs_sweep2_prolong_*, on FP64
  * and FP32 levels, default; 1: on FP64 levels only; 0: mgp_prolong_add_* and

@@ -140,6 +140,7 @@ int mgp_tuning_set(const char* key, int64_t value) {
         mgp::g_sweep2_rows = value;
     else if (!strcmp(key, "sweep2.prolong") && value <= 2) mgp::g_sweep2_prolong = value;
     else if (!strcmp(key, "sweep2.pack") && value <= 3) mgp::g_sweep2_pack = value;
+    else if (!strcmp(key, "sweep2.target_waves") && value >= 1 && value <= (1 << 20)) mgp::g_sweep2_waves = value;
     else if (!strcmp(key, "residual_fw.rows") && (value == 0 || value == 4 || value == 6)) mgp::g_fw_rows = value;
     else if (!strcmp(key, "residual_fw.depth") && (value == 0 || value == 1 || value == 2)) mgp::g_fw_depth = value;
     else if (!strcmp(key, "batch.blocks") && value >= 1 && value <= (1 << 20)) mgp::g_batch_blocks = value;
@@ -171,6 +172,7 @@ int mgp_tuning_get(const char* key, int64_t* value) {
     else if (!strcmp(key, "sweep2.rows")) *value = mgp::g_sweep2_rows;
     else if (!strcmp(key, "sweep2.prolong")) *value = mgp::g_sweep2_prolong;
     else if (!strcmp(key, "sweep2.pack")) *value = mgp::g_sweep2_pack;
+    else if (!strcmp(key, "sweep2.target_waves")) *value = mgp::g_sweep2_waves;
     else if (!strcmp(key, "residual_fw.rows")) *value = mgp::g_fw_rows;
     else if (!strcmp(key, "residual_fw.depth")) *value = mgp::g_fw_depth;
     else if (!strcmp(key, "batch.blocks")) *value = mgp::g_batch_blocks;

@@ -25,6 +25,7 @@ extern int64_t g_sweep_pair;
 extern int64_t g_sweep2_rows;
 extern int64_t g_sweep2_prolong;
 extern int64_t g_sweep2_pack;
+extern int64_t g_sweep2_waves;
 // residual on a box through wave tiles (norm: max |r| over the whole shape
 // folded into *norm); MGP_ERR_ARG when the layouts do not allow it
 template <class T>

@@ -2434,6 +2434,7 @@ int64_t g_sweep2_rows = 0;  // register rows per two-sweep tile: FP64 12 or 16, 
 // FP64, bit 1 FP32).  1025 x 1025 x 513: FP64 4.53 ms against 4.92 ms per pass,
 // with the correction inside 5.71 against 5.80 ms; FP32 2.29 against 2.48 ms
 int64_t g_sweep2_pack = 3;
+int64_t g_sweep2_waves = 6800;  // least waves per two-sweep launch (x-chunks added to reach it)
 int64_t g_sweep2_prolong = 2;  // the cycles add the prolongation inside the first post-smoothing pass
                                // (1: FP64 levels, 2: FP32 levels too)
 
@@ -2470,13 +2471,13 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
     const int64_t tiles = (int64_t)A.ytiles * A.ztiles;
     const int64_t nx = b[1] - b[0];
     // x chunks per (y, z) tile (each chunk spends 6 warm-up steps): at least
-    // sweep2.chunks_per_tile, enough for about 6800 waves on coarser levels,
+    // sweep2.chunks_per_tile, enough for sweep2.target_waves waves on coarser levels,
     // and an odd count.  Measured, FP64: 1025 x 1025 x 513 with 13, 15, 17
     // chunks 4.94-5.01 ms, with 12, 14, 16, 18 chunks 5.19-5.21 ms;
     // 513 x 513 x 257 with 21 chunks 0.82 ms, with 16 0.89 ms; 257 x 257 x 129
     // and below flat.
     const int64_t tl = std::max<int64_t>(tiles, 1);
-    int64_t chunks = std::max<int64_t>(g_sweep2_cpt, (6800 + tl - 1) / tl) | 1;
+    int64_t chunks = std::max<int64_t>(g_sweep2_cpt, (g_sweep2_waves + tl - 1) / tl) | 1;
     chunks = std::min<int64_t>(chunks, std::max<int64_t>(1, nx / std::max<int64_t>(min_chunk_for(box_points(b)), 1)));
     A.xchunk = (nx + chunks - 1) / chunks;
     A.ntiles = (int)(tiles * ((nx + A.xchunk - 1) / A.xchunk));
