@@ -194,6 +194,7 @@ def main():
     log("bench: sweep %.2f ms (%.0f GB/s), two-sweep pass %.2f ms (%.0f GB/s), with the correction %.2f ms "
         "(%.0f GB/s), residual %.2f ms (%.0f GB/s)" %
         (sweep_ms, sweep_gbs, pair_ms, pair_gbs, pro_ms, pro_gbs, resid_ms, resid_gbs))
+    f32_kernels = fp32_pass_times(u, u1, F, bx, box_pts, stream, args.kernel_reps)
     # fine-level launches per F-cycle: three smoothing passes, the two after a
     # correction with the correction inside where sweep2.prolong is on
     fused = ctypes.c_int64(0)
@@ -247,7 +248,8 @@ def main():
                     "rbgs_sweep2_prolong": {"ms": pro_ms, "GB/s": pro_gbs, "frac": pro_gbs / HBM_PEAK_GBS,
                                             "launches_per_cycle": n_pro},
                     "rbgs_sweep": {"ms": sweep_ms, "GB/s": sweep_gbs, "frac": sweep_gbs / HBM_PEAK_GBS},
-                    "residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS}},
+                    "residual_norm": {"ms": resid_ms, "GB/s": resid_gbs, "frac": resid_gbs / HBM_PEAK_GBS},
+                    **f32_kernels},
         "fp32_refine": mixed,
         "batched": batch,
         "serving": serve,
@@ -255,6 +257,52 @@ def main():
         "cpu_port": port,
     }
     print(json.dumps(out), flush=True)
+
+
+def fp32_pass_times(u, u1, F, bx, box_pts, stream, reps):
+    """The FP32 two-sweep passes of the mixed-precision cycles on arrays of the
+    fine level's layout (u, f, an output and level 1's u, FP32, homogeneous
+    Neumann data like the correction problem's): the plain pass (12 B per box
+    point) and the pass with the correction inside (12 B per box point + 4 B per
+    level-1 point), each timed like the FP64 kernels."""
+    import numpy as np
+    L = mgp.lib()
+    shape = [u.shape[q] for q in range(3)]
+    halo = [u.halo[q] for q in range(6)]
+    cshape = [u1.shape[q] for q in range(3)]
+    chalo = [u1.halo[q] for q in range(6)]
+    a, b, f = (mgp.DeviceArray(shape, halo, np.float32) for _ in range(3))
+    c = mgp.DeviceArray(cshape, chalo, np.float32)
+    for arr, v in ((a, 0.25), (b, 0.0), (f, 1.0), (c, 0.0)):
+        mgp.check(L.mgp_fill_f32(arr.ref(), v, None))
+    F32 = mgp.Faces()
+    keep = []
+    for q in range(6):
+        F32.type[q] = F.type[q]
+        if F.type[q] == mgp.NEUMANN:
+            m = list(shape)
+            m[{0: 1, 1: 1, 2: 0, 3: 0, 4: 2, 5: 2}[q]] = 1
+            g = mgp.DeviceBuffer(np.zeros(int(np.prod(m)), np.float32))
+            keep.append(g)
+            F32.g[q] = g.ptr
+    F32.h = F.h
+    R = ctypes.byref
+    plain = kernel_time_ms(lambda st: mgp.check(L.mgp_rbgs_sweep2_f32(mgp.CONSISTENT, a.ref(), b.ref(), f.ref(),
+                                                                       R(F32), bx, 1.0, st)), reps, stream)
+    fused = kernel_time_ms(lambda st: mgp.check(L.mgp_rbgs_sweep2_prolong_f32(mgp.CONSISTENT, a.ref(), b.ref(),
+                                                                               f.ref(), R(F32), bx, 1.0, c.ref(),
+                                                                               st)), reps, stream)
+    L.mgp_stream_sync(stream)
+    cpts = cshape[0] * cshape[1] * cshape[2]
+    out = {}
+    for name, ms, nbytes in (("rbgs_sweep2_f32", plain, 12.0 * box_pts),
+                             ("rbgs_sweep2_prolong_f32", fused, 12.0 * box_pts + 4.0 * cpts)):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        out[name] = {"ms": ms, "GB/s": gbs, "frac": gbs / HBM_PEAK_GBS}
+    log("bench: FP32 two-sweep pass %.2f ms, with the correction %.2f ms" % (plain, fused))
+    for arr in (a, b, f, c):
+        arr.free()
+    return out
 
 
 def measured_traffic(n, kernel):
