@@ -1086,6 +1086,24 @@ __global__ void __launch_bounds__(64) k_rbgs2_wave(MarchArgs<T> A) {
         wave_march2<T, MODE, NR, false, false, 0, 0, PACK, false, FLDS>(A, tl, nullptr, fr);
 }
 
+// packed stages, f in LDS, held to two waves per SIMD (consistent mode; the
+// row counts other than 16)
+template <class T, int NR>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(2, 2))) k_rbgs2f_wave(MarchArgs<T> A) {
+    if ((int)blockIdx.x >= A.ntiles_pad) {
+        copy_complement(A, (int)blockIdx.x - A.ntiles_pad);
+        return;
+    }
+    const int per_xcd = A.ntiles_pad / 8;
+    const int tl = (int)(blockIdx.x % 8) * per_xcd + (int)(blockIdx.x / 8);
+    if (tl >= A.ntiles) return;
+    __shared__ T fr[4 * (NR - 2) * 64];
+    if (sweep2_edge_tile<T, NR>(A, tl))
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, true, false, 0, 0, true, false, true>(A, tl, nullptr, fr);
+    else
+        wave_march2<T, MGP_MODE_CONSISTENT, NR, false, false, 0, 0, true, false, true>(A, tl, nullptr, fr);
+}
+
 // the two sweeps of u = 0 (consistent mode): u_in is not read, the shape
 // points outside the box are written 0
 template <class T, int NR, bool PACK>
@@ -2510,10 +2528,17 @@ int march_sweep2(int mode, const mgp_array* ui, const mgp_array* uo, const mgp_a
         const hipError_t e = hipGetLastError();
         return e == hipSuccess ? MGP_OK : (int)e;
     }
-    // 12 rows, packed, f in LDS: two waves per SIMD for FP64 (238 VGPRs), yet
-    // 4.62 against 4.53 ms with 16 rows at 1025 x 1025 x 513 (sweep2.rows = 12)
-    if (rows == 12 && (g_sweep2_pack & (sizeof(T) == 8 ? 1 : 2)) && mode == MGP_MODE_CONSISTENT) {
-        hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_CONSISTENT, 12, true, true>), g, dim3(64), 0, st, A);
+    // other row counts, packed, f in LDS (sweep2.rows): FP64 12 rows and FP32 20
+    // rows run two waves per SIMD, and take what 16 rows at one wave take at
+    // 1025 x 1025 x 513 (4.54 against 4.53 ms, 2.32 against 2.30 ms); FP32 24
+    // rows 2.65 ms
+    if (rows != 16 && (g_sweep2_pack & (sizeof(T) == 8 ? 1 : 2)) && mode == MGP_MODE_CONSISTENT) {
+        if constexpr (sizeof(T) == 8) {
+            hipLaunchKernelGGL((k_rbgs2f_wave<T, 12>), g, dim3(64), 0, st, A);
+        } else {
+            if (rows == 24) hipLaunchKernelGGL((k_rbgs2_wave<T, MGP_MODE_CONSISTENT, 24, true, true>), g, dim3(64), 0, st, A);
+            else hipLaunchKernelGGL((k_rbgs2f_wave<T, 20>), g, dim3(64), 0, st, A);
+        }
         const hipError_t e = hipGetLastError();
         return e == hipSuccess ? MGP_OK : (int)e;
     }

@@ -550,7 +550,7 @@ SWEEP2_SHAPES = SWEEP_SHAPES + [((3, 40, 200), "NNDDNN"), ((21, 3, 3), "NNNNNN")
 # (dtype, sweep2.rows, sweep.target_wgs): several x-chunks per tile, and one
 SWEEP2_CASES = [("f64", 16, 16384), ("f64", 12, 16384), ("f64", 16, 1), ("f32", 20, 16384), ("f32", 16, 16384),
                 ("f32", 24, 1), ("f64p", 16, 16384), ("f64p", 16, 1), ("f32p", 16, 16384), ("f32p", 16, 1),
-                ("f64p", 12, 16384), ("f64p", 12, 1)]
+                ("f64p", 12, 16384), ("f64p", 12, 1), ("f32p", 20, 16384), ("f32p", 24, 1)]
 
 
 @contextlib.contextmanager
