@@ -766,6 +766,10 @@ __global__ void __launch_bounds__(64) k_rbgs_wave(MarchArgs<T> A) {
 // (z >> 1) + 1 on the lane after.  Rows, lanes and planes outside the shape
 // receive values that no update reads (they are masked, or ghosts formed
 // afresh before every stage).
+//
+// ZIN (mgp_rbgs_sweep2_from_zero_*): the input planes are zeros and are not
+// read; the passes' arithmetic is unchanged, so the result is that of a fill of
+// 0 followed by the pass.  PACK: the packed stages (stage lambda below).
 constexpr int WZ2 = 56;  // interior columns per two-sweep wave tile
 
 template <class T, int MODE, int NR, bool EDGE, bool PRO = false, int PY = 0, int PZ = 0, bool PACK = false,
